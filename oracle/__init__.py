@@ -1,0 +1,40 @@
+"""FP64 CPU oracle for the ns2d / ns3d / ns3d.strat pseudo-spectral time step.
+
+TEST INFRASTRUCTURE ONLY. This package is a plain numpy/scipy restatement of
+fluidsim's algorithm for this path (each function cites the reference file:line
+it follows). Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
+as the timed CPU baseline -- never as a fallback for the GPU engine.
+
+Pinning (what fixes each function without importing fluidsim, whose
+dependencies are absent here): see the pin column below and the tests in
+``tests/test_oracle_*.py``.
+
+==========================  ===================================================
+function                    pinned by
+==========================  ===================================================
+Oper2D/Oper3D.fft/ifft      brute-force DFT at n<=12, Parseval, round trip
+sum_wavenumbers             Parseval identity (fluidsim/base/state.py:385-392)
+curl / vecfft / grad        round trip rtol 1e-15 (test_operators2d.py:69-80),
+                            single-mode closed forms
+project_perpk3d/perp        k.P(v)=0, idempotence, single-mode closed form
+project_toroidal/poloidal   radial+poloidal+toroidal = id (test_operators3d.py
+                            :353-480)
+dealiasing masks            index predicate, brute force on small grids
+ns2d tendency               energy/enstrophy conservation (ns2d/test_solver.py
+                            :56-69, 181-193), Taylor-Green N=0, single-mode
+                            Jacobian closed form, planted-mistake tests
+ns3d tendency               energy conservation (ns3d/test_solver.py:80-96),
+                            ABC flow N=0, closed-form two-mode product
+strat tendency              energy incl. E_A conservation (strat/test_solver.py
+                            :35-58), linear wave closed form
+IF-RK4 / RK2                N=0 exact decay, 4th/2nd order self-convergence,
+                            linear ODE closed form per stage
+CFL                         scripted maxima, hysteresis rule
+energy budget               dE/dt = -eps to O(dt^4)
+spectra binning             parity unpinned (fluidfft binning not in tree); sum
+                            rule sum(E_k)*dk = E pins the normalisation only
+==========================  ===================================================
+"""
+
+from .operators import Oper2D, Oper3D  # noqa: F401
