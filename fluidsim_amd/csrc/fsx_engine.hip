@@ -1,0 +1,794 @@
+// Plans, launches and the C-ABI of the fluidsim_amd engine (gfx950).
+//
+// The exported functions are declared in include/fluidsim_amd.h.  Every
+// function returns 0 on success or a negative code; fsx_last_error() gives
+// the message.  Device buffers are plain pointers (complex = interleaved
+// re/im of the plan precision), streams are hipStream_t passed as void*.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fluidsim_amd.h"
+#include "fsx_fft.h"
+#include "fsx_kernels.h"
+#include "fsx_pointwise.h"
+
+using namespace fsx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define FSX_HIP(call)                                                      \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return fail(FSX_ERR_HIP, std::string(#call) + ": " +                 \
+                                   hipGetErrorString(e_));                 \
+  } while (0)
+
+int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+struct fsx_plan_s {
+  int dim, prec;
+  int nx, ny, nz, nkx;
+  double Lx, Ly, Lz, dkx, dky, dkz;
+  AxisMap ymF, zmF;  // all modes
+  int mxF;
+  AxisMap ymP, zmP;  // bounding box of the non-dealiased modes
+  int mxP;
+  unsigned char* d_mask;  // where_dealiased when the kept set is not a box
+  bool dealias;
+  void* tw[3];  // x, y, z twiddle tables
+  void* w1;
+  void* w2;
+  size_t w1_elems, w2_elems;  // capacity in complex elements
+  int nf_max;
+  double* d_scal;  // [0..3] maxima bits, [4] dt, [5] nan flag, [8..15] sums
+  double* d_spec;
+  int spec_cap;
+};
+
+namespace {
+
+constexpr int kEPT32 = 32;  // 8192 complex64 per tile  = 64 KiB of LDS
+constexpr int kEPT64 = 16;  // 4096 complex128 per tile = 64 KiB of LDS
+
+template <typename T>
+constexpr int ept() {
+  return sizeof(T) == 4 ? kEPT32 : kEPT64;
+}
+
+template <typename T>
+int upload_twiddles(int n, void** out) {
+  std::vector<cpx<T>> h(n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < n; ++m) {
+    const long double a = two_pi * (long double)m / (long double)n;
+    h[m].x = (T)cosl(a);
+    h[m].y = (T)(-sinl(a));
+  }
+  FSX_HIP(hipMalloc(out, n * sizeof(cpx<T>)));
+  FSX_HIP(hipMemcpy(*out, h.data(), n * sizeof(cpx<T>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T, int DIR, class Op>
+int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw,
+                   hipStream_t s) {
+  constexpr int E = ept<T>();
+  const int tile = E * kThreads;
+  if (N > tile) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
+  int L = tile / N;
+  int cap = 1;
+  while (cap < nlines) cap <<= 1;
+  if (L > cap) L = cap;
+  op.d.N = N;
+  op.d.logN = ilog2(N);
+  op.d.L = L;
+  op.d.logL = ilog2(L);
+  op.d.nouter = nouter;
+  op.d.nlines = nlines;
+  op.d.nf = nf;
+  op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
+  if (nouter <= 0 || nlines <= 0 || nf <= 0) return 0;
+  const long long nchunks = (nlines + L - 1) / L;
+  const long long blocks = nchunks * nouter;
+  if (blocks > 0x7fffffffLL) return fail(FSX_ERR_ARG, "grid too large");
+  const size_t lds = (size_t)tile * sizeof(cpx<T>);
+  auto kern = k_strided<T, DIR, E, Op>;
+  static bool attr = false;
+  if (!attr) {
+    FSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    attr = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, s, op);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T, class Op>
+int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
+                 int mx_out, T scl, const void* tw, hipStream_t s) {
+  constexpr int E = ept<T>();
+  const int tile = E * kThreads;
+  const int M = nx / 2;
+  const int nfm = nf_in > nf_out ? nf_in : nf_out;
+  if (nfm * M > tile) return fail(FSX_ERR_ARG, "row pass exceeds the tile");
+  int RB = tile / (nfm * M);
+  if (RB > nrows) RB = nrows;
+  if (RB < 1) RB = 1;
+  op.d.nx = nx;
+  op.d.M = M;
+  op.d.logM = ilog2(M);
+  op.d.RB = RB;
+  op.d.nrows = nrows;
+  op.d.nf_in = nf_in;
+  op.d.nf_out = nf_out;
+  op.d.mx_in = mx_in;
+  op.d.mx_out = mx_out;
+  op.d.scale = scl;
+  op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
+  if (nrows <= 0) return 0;
+  const long long blocks = (nrows + RB - 1) / RB;
+  const size_t lds = (size_t)tile * sizeof(cpx<T>);
+  auto kern = k_xpass<T, E, Op>;
+  static bool attr = false;
+  if (!attr) {
+    FSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    attr = true;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, s, op);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+Box make_box(const fsx_plan_s* p, bool pruned) {
+  Box b;
+  b.dim = p->dim;
+  b.nx = p->nx;
+  b.ny = p->ny;
+  b.nz = p->nz;
+  b.nkx = p->nkx;
+  b.mx = pruned ? p->mxP : p->mxF;
+  b.ym = pruned ? p->ymP : p->ymF;
+  b.zm = pruned ? p->zmP : p->zmF;
+  b.dkx = p->dkx;
+  b.dky = p->dky;
+  b.dkz = p->dkz;
+  return b;
+}
+
+unsigned grid_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+template <typename T>
+cpx<T>* W1(fsx_plan_s* p, int f, bool pruned) {
+  const size_t fe = pruned ? (size_t)p->nz * p->ymP.nk() * p->mxP
+                           : (size_t)p->nz * p->ny * p->nkx;
+  return reinterpret_cast<cpx<T>*>(p->w1) + f * fe;
+}
+template <typename T>
+cpx<T>* W2(fsx_plan_s* p, int f, bool pruned) {
+  const size_t fe = (size_t)p->nz * p->ny * (pruned ? p->mxP : p->nkx);
+  return reinterpret_cast<cpx<T>*>(p->w2) + f * fe;
+}
+
+// ---------------------------------------------------------------- inverse
+// K (nf fields, stride Kelems) -> W2 (pruned or full) through z and y passes.
+template <typename T>
+int inverse_to_w2(fsx_plan_s* p, const cpx<T>* const* kin, int nf, bool pruned,
+                  hipStream_t s) {
+  const AxisMap ym = pruned ? p->ymP : p->ymF;
+  const AxisMap zm = pruned ? p->zmP : p->zmF;
+  const int mx = pruned ? p->mxP : p->mxF;
+  if (p->dim == 2) {
+    OpYInv2<T> op;
+    for (int f = 0; f < nf; ++f) {
+      op.in[f] = kin[f];
+      op.out[f] = W2<T>(p, f, pruned);
+    }
+    op.nkx = p->nkx;
+    op.mx = mx;
+    op.ym = ym;
+    return launch_strided<T, +1>(op, p->ny, 1, mx, nf, p->tw[1], s);
+  }
+  {
+    OpZInv<T> op;
+    for (int f = 0; f < nf; ++f) {
+      op.in[f] = kin[f];
+      op.out[f] = W1<T>(p, f, pruned);
+    }
+    op.ny = p->ny;
+    op.nkx = p->nkx;
+    op.mx = mx;
+    op.ym = ym;
+    op.zm = zm;
+    int rc = launch_strided<T, +1>(op, p->nz, ym.nk(), mx, nf, p->tw[2], s);
+    if (rc) return rc;
+  }
+  OpYInv3<T> op;
+  for (int f = 0; f < nf; ++f) {
+    op.in[f] = W1<T>(p, f, pruned);
+    op.out[f] = W2<T>(p, f, pruned);
+  }
+  op.ny = p->ny;
+  op.mx = mx;
+  op.ym = ym;
+  return launch_strided<T, +1>(op, p->ny, p->nz, mx, nf, p->tw[1], s);
+}
+
+// W2 (nf fields) -> K through y and z passes; only kept modes are written.
+template <typename T>
+int forward_from_w2(fsx_plan_s* p, cpx<T>* const* kout, int nf, bool pruned,
+                    hipStream_t s) {
+  const AxisMap ym = pruned ? p->ymP : p->ymF;
+  const AxisMap zm = pruned ? p->zmP : p->zmF;
+  const int mx = pruned ? p->mxP : p->mxF;
+  const unsigned char* mask = pruned ? p->d_mask : nullptr;
+  if (p->dim == 2) {
+    OpYFwd2<T> op;
+    for (int f = 0; f < nf; ++f) {
+      op.in[f] = W2<T>(p, f, pruned);
+      op.out[f] = kout[f];
+    }
+    op.nkx = p->nkx;
+    op.mx = mx;
+    op.ym = ym;
+    op.mask = mask;
+    return launch_strided<T, -1>(op, p->ny, 1, mx, nf, p->tw[1], s);
+  }
+  {
+    OpYFwd3<T> op;
+    for (int f = 0; f < nf; ++f) {
+      op.in[f] = W2<T>(p, f, pruned);
+      op.out[f] = W1<T>(p, f, pruned);
+    }
+    op.ny = p->ny;
+    op.mx = mx;
+    op.ym = ym;
+    int rc = launch_strided<T, -1>(op, p->ny, p->nz, mx, nf, p->tw[1], s);
+    if (rc) return rc;
+  }
+  OpZFwd<T> op;
+  for (int f = 0; f < nf; ++f) {
+    op.in[f] = W1<T>(p, f, pruned);
+    op.out[f] = kout[f];
+  }
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.mx = mx;
+  op.ym = ym;
+  op.zm = zm;
+  op.mask = mask;
+  return launch_strided<T, -1>(op, p->nz, ym.nk(), mx, nf, p->tw[2], s);
+}
+
+size_t kelems(const fsx_plan_s* p) { return (size_t)p->nz * p->ny * p->nkx; }
+size_t xelems(const fsx_plan_s* p) { return (size_t)p->nz * p->ny * p->nx; }
+int nrows(const fsx_plan_s* p) { return p->nz * p->ny; }
+
+template <typename T>
+int do_ifft(fsx_plan_s* p, const void* kin, void* xout, int nfields,
+            hipStream_t s) {
+  for (int f = 0; f < nfields; ++f) {
+    const cpx<T>* k = reinterpret_cast<const cpx<T>*>(kin) + f * kelems(p);
+    T* x = reinterpret_cast<T*>(xout) + f * xelems(p);
+    int rc = inverse_to_w2<T>(p, &k, 1, false, s);
+    if (rc) return rc;
+    OpXC2R<T> op;
+    op.in[0] = W2<T>(p, 0, false);
+    op.outr[0] = x;
+    rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, p->mxF, T(1),
+                         p->tw[0], s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+template <typename T>
+int do_fft(fsx_plan_s* p, const void* xin, void* kout, int nfields,
+           hipStream_t s) {
+  const double npts = (double)p->nx * p->ny * p->nz;
+  for (int f = 0; f < nfields; ++f) {
+    const T* x = reinterpret_cast<const T*>(xin) + f * xelems(p);
+    cpx<T>* k = reinterpret_cast<cpx<T>*>(kout) + f * kelems(p);
+    OpXR2C<T> op;
+    op.in[0] = x;
+    op.out[0] = W2<T>(p, 0, false);
+    int rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, p->mxF,
+                             T(1.0 / npts), p->tw[0], s);
+    if (rc) return rc;
+    rc = forward_from_w2<T>(p, &k, 1, false, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+template <typename T>
+int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
+            int want_maxima, hipStream_t s) {
+  OpYInv2NS<T> op;
+  op.in[0] = reinterpret_cast<const cpx<T>*>(rot);
+  for (int f = 0; f < 4; ++f) op.out[f] = W2<T>(p, f, true);
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.mx = p->mxP;
+  op.ym = p->ymP;
+  op.dkx = (T)p->dkx;
+  op.dky = (T)p->dky;
+  int rc = launch_strided<T, +1>(op, p->ny, 1, p->mxP, 4, p->tw[1], s);
+  if (rc) return rc;
+  OpXNS2D<T> ox;
+  for (int f = 0; f < 4; ++f) ox.in[f] = W2<T>(p, f, true);
+  ox.out[0] = W2<T>(p, 0, true);
+  ox.maxima =
+      want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
+  ox.beta = (T)beta;
+  const double npts = (double)p->nx * p->ny;
+  rc = launch_xpass<T>(ox, p->nx, p->ny, 4, 1, p->mxP, p->mxP, T(1.0 / npts),
+                       p->tw[0], s);
+  if (rc) return rc;
+  cpx<T>* k = reinterpret_cast<cpx<T>*>(out);
+  return forward_from_w2<T>(p, &k, 1, true, s);
+}
+
+template <typename T>
+int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
+            int want_maxima, hipStream_t s) {
+  const size_t ke = kelems(p);
+  OpZInvCurl<T> op;
+  for (int f = 0; f < 3; ++f)
+    op.in[f] = reinterpret_cast<const cpx<T>*>(v) + f * ke;
+  for (int f = 0; f < 6; ++f) op.out[f] = W1<T>(p, f, true);
+  op.ny = p->ny;
+  op.nz = p->nz;
+  op.nkx = p->nkx;
+  op.mx = p->mxP;
+  op.ym = p->ymP;
+  op.zm = p->zmP;
+  op.dkx = (T)p->dkx;
+  op.dky = (T)p->dky;
+  op.dkz = (T)p->dkz;
+  op.has_f = has_f;
+  op.fcor = (T)fcor;
+  int rc = launch_strided<T, +1>(op, p->nz, p->ymP.nk(), p->mxP, 6, p->tw[2], s);
+  if (rc) return rc;
+  OpYInv3<T> oy;
+  for (int f = 0; f < 6; ++f) {
+    oy.in[f] = W1<T>(p, f, true);
+    oy.out[f] = W2<T>(p, f, true);
+  }
+  oy.ny = p->ny;
+  oy.mx = p->mxP;
+  oy.ym = p->ymP;
+  rc = launch_strided<T, +1>(oy, p->ny, p->nz, p->mxP, 6, p->tw[1], s);
+  if (rc) return rc;
+  OpXNS3D<T> ox;
+  for (int f = 0; f < 6; ++f) ox.in[f] = W2<T>(p, f, true);
+  for (int f = 0; f < 3; ++f) ox.out[f] = W2<T>(p, f, true);
+  ox.maxima =
+      want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
+  const double npts = (double)p->nx * p->ny * p->nz;
+  rc = launch_xpass<T>(ox, p->nx, nrows(p), 6, 3, p->mxP, p->mxP,
+                       T(1.0 / npts), p->tw[0], s);
+  if (rc) return rc;
+  cpx<T>* k[3];
+  for (int f = 0; f < 3; ++f) k[f] = reinterpret_cast<cpx<T>*>(out) + f * ke;
+  return forward_from_w2<T>(p, k, 3, true, s);
+}
+
+template <typename T>
+int do_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S, void* tmp,
+          void* t1, void* N, int project, int no_vz_kz0, const double* visc,
+          hipStream_t s) {
+  RKArgs<T> a;
+  const size_t ke = kelems(p);
+  for (int v = 0; v < nvar; ++v) {
+    a.S[v] = reinterpret_cast<cpx<T>*>(S) + v * ke;
+    a.tmp[v] = tmp ? reinterpret_cast<cpx<T>*>(tmp) + v * ke : nullptr;
+    a.t1[v] = reinterpret_cast<cpx<T>*>(t1) + v * ke;
+    a.Nt[v] = reinterpret_cast<cpx<T>*>(N) + v * ke;
+  }
+  a.nvar = nvar;
+  a.scheme = scheme;
+  a.stage = stage;
+  a.project = project;
+  a.no_vz_kz0 = no_vz_kz0;
+  a.dt = p->d_scal + 4;
+  a.visc.nu2 = visc[0];
+  a.visc.nu4 = visc[1];
+  a.visc.nu8 = visc[2];
+  a.visc.num4 = visc[3];
+  a.box = make_box(p, true);
+  a.nanflag = reinterpret_cast<unsigned int*>(p->d_scal + 5);
+  const size_t n = (size_t)(p->dim == 3 ? p->zmP.nk() : 1) * p->ymP.nk() * p->mxP;
+  hipLaunchKernelGGL(k_rk_stage<T>, dim3(grid_for(n)), dim3(256), 0, s, a);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int do_sums(fsx_plan_s* p, int kind, const void* f0, const void* f1,
+            const void* f2, const void* f3, double N2, const double* visc,
+            double* host_out, hipStream_t s) {
+  Visc v = {visc[0], visc[1], visc[2], visc[3]};
+  double* d = p->d_scal + 8;
+  FSX_HIP(hipMemsetAsync(d, 0, 6 * sizeof(double), s));
+  const size_t n = kelems(p);
+  hipLaunchKernelGGL(k_sums<T>, dim3(grid_for(n)), dim3(256), 0, s,
+                     reinterpret_cast<const cpx<T>*>(f0),
+                     reinterpret_cast<const cpx<T>*>(f1),
+                     reinterpret_cast<const cpx<T>*>(f2),
+                     reinterpret_cast<const cpx<T>*>(f3), kind, N2,
+                     make_box(p, false), v, d);
+  FSX_HIP(hipGetLastError());
+  FSX_HIP(hipMemcpyAsync(host_out, d, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int fsx_version(void) { return FSX_ABI_VERSION; }
+
+int fsx_last_error(char* buf, int len) {
+  if (!buf || len <= 0) return -1;
+  std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return 0;
+}
+
+int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
+                    double Ly, double Lz, int precision,
+                    const unsigned char* where_dealiased, int nf_max) {
+  if (!out) return fail(FSX_ERR_ARG, "null plan pointer");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) return fail(FSX_ERR_ARG, "dim must be 2 or 3");
+  if (dim == 2) nz = 1;
+  if (precision != 4 && precision != 8)
+    return fail(FSX_ERR_ARG, "precision must be 4 (float32) or 8 (float64)");
+  if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz) || nx < 4 || ny < 2 ||
+      (dim == 3 && nz < 2))
+    return fail(FSX_ERR_ARG,
+                "the engine supports power-of-two grids (nx >= 4, ny, nz >= 2)");
+  const int tile = (precision == 4 ? kEPT32 : kEPT64) * kThreads;
+  if (ny > tile || nz > tile || nx / 2 > tile)
+    return fail(FSX_ERR_ARG, "grid dimension too large for the LDS tile");
+  if (nf_max < 1 || nf_max > 8) return fail(FSX_ERR_ARG, "nf_max must be 1..8");
+  if (!(Lx > 0) || !(Ly > 0) || (dim == 3 && !(Lz > 0)))
+    return fail(FSX_ERR_ARG, "box lengths must be positive");
+
+  fsx_plan_s* p = new fsx_plan_s();
+  std::memset(p, 0, sizeof(*p));
+  p->dim = dim;
+  p->prec = precision;
+  p->nx = nx;
+  p->ny = ny;
+  p->nz = nz;
+  p->nkx = nx / 2 + 1;
+  p->Lx = Lx;
+  p->Ly = Ly;
+  p->Lz = dim == 3 ? Lz : 1.0;
+  const double two_pi = 6.283185307179586476925286766559005768;
+  p->dkx = two_pi / Lx;
+  p->dky = two_pi / Ly;
+  p->dkz = dim == 3 ? two_pi / Lz : 0.0;
+  p->ymF = AxisMap{ny, ny / 2 + 1, ny - (ny / 2 + 1)};
+  p->zmF = dim == 3 ? AxisMap{nz, nz / 2 + 1, nz - (nz / 2 + 1)} : AxisMap{1, 1, 0};
+  p->mxF = p->nkx;
+  p->nf_max = nf_max;
+
+  // bounding box of the kept (non-dealiased) modes
+  p->ymP = p->ymF;
+  p->zmP = p->zmF;
+  p->mxP = p->mxF;
+  if (where_dealiased) {
+    const int nkx = p->nkx;
+    std::vector<char> kx_any(nkx, 0), ky_any(ny, 0), kz_any(nz, 0);
+    size_t nkept = 0;
+    for (int iz = 0; iz < nz; ++iz)
+      for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nkx; ++ix)
+          if (!where_dealiased[((size_t)iz * ny + iy) * nkx + ix]) {
+            kx_any[ix] = ky_any[iy] = kz_any[iz] = 1;
+            ++nkept;
+          }
+    int mx = 0;
+    for (int ix = 0; ix < nkx; ++ix)
+      if (kx_any[ix]) mx = ix + 1;
+    auto axis = [](const std::vector<char>& any, int n) {
+      int lo = 0, hi = 0;
+      for (int i = 0; i <= n / 2; ++i)
+        if (any[i]) lo = i + 1;
+      for (int i = n - 1; i > n / 2; --i)
+        if (any[i]) hi = n - i;
+      return AxisMap{n, lo, hi};
+    };
+    if (nkept == 0) mx = 1;
+    p->mxP = mx > 0 ? mx : 1;
+    p->ymP = axis(ky_any, ny);
+    if (p->ymP.nk() == 0) p->ymP = AxisMap{ny, 1, 0};
+    p->zmP = dim == 3 ? axis(kz_any, nz) : AxisMap{1, 1, 0};
+    if (p->zmP.nk() == 0) p->zmP = AxisMap{nz, 1, 0};
+    const size_t box = (size_t)p->zmP.nk() * p->ymP.nk() * p->mxP;
+    p->dealias = true;
+    if (box != nkept) {
+      const size_t n = (size_t)nz * ny * nkx;
+      hipError_t e = hipMalloc((void**)&p->d_mask, n);
+      if (e == hipSuccess)
+        e = hipMemcpy(p->d_mask, where_dealiased, n, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        fsx_plan_destroy(p);
+        return fail(FSX_ERR_HIP, std::string("mask upload: ") + hipGetErrorString(e));
+      }
+    }
+  }
+
+  int rc = 0;
+  const int ns[3] = {nx, ny, nz};
+  for (int a = 0; a < (dim == 3 ? 3 : 2) && !rc; ++a)
+    rc = precision == 4 ? upload_twiddles<float>(ns[a], &p->tw[a])
+                        : upload_twiddles<double>(ns[a], &p->tw[a]);
+  if (rc) {
+    fsx_plan_destroy(p);
+    return rc;
+  }
+  const size_t full = (size_t)nz * ny * p->nkx;
+  const size_t w1p = (size_t)nf_max * nz * p->ymP.nk() * p->mxP;
+  const size_t w2p = (size_t)nf_max * nz * ny * p->mxP;
+  p->w1_elems = dim == 3 ? (w1p > full ? w1p : full) : 0;
+  p->w2_elems = w2p > full ? w2p : full;
+  const size_t csz = 2 * (size_t)precision;
+  hipError_t e = hipSuccess;
+  if (p->w1_elems) e = hipMalloc(&p->w1, p->w1_elems * csz);
+  if (e == hipSuccess) e = hipMalloc(&p->w2, p->w2_elems * csz);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_scal, 32 * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(p->d_scal, 0, 32 * sizeof(double));
+  if (e != hipSuccess) {
+    fsx_plan_destroy(p);
+    return fail(FSX_ERR_HIP, std::string("workspace: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return 0;
+}
+
+int fsx_plan_destroy(fsx_plan p) {
+  if (!p) return 0;
+  for (int a = 0; a < 3; ++a)
+    if (p->tw[a]) (void)hipFree(p->tw[a]);
+  if (p->w1) (void)hipFree(p->w1);
+  if (p->w2) (void)hipFree(p->w2);
+  if (p->d_mask) (void)hipFree(p->d_mask);
+  if (p->d_scal) (void)hipFree(p->d_scal);
+  if (p->d_spec) (void)hipFree(p->d_spec);
+  delete p;
+  return 0;
+}
+
+int fsx_plan_info(fsx_plan p, int* info, int n) {
+  if (!p || !info) return fail(FSX_ERR_ARG, "null argument");
+  const int v[12] = {p->dim, p->prec, p->nx, p->ny, p->nz, p->nkx,
+                     p->mxP, p->ymP.lo, p->ymP.hi, p->zmP.lo, p->zmP.hi,
+                     p->d_mask != nullptr};
+  for (int i = 0; i < n && i < 12; ++i) info[i] = v[i];
+  return 0;
+}
+
+int fsx_fft(fsx_plan p, const void* x, void* x_fft, int nfields, void* stream) {
+  if (!p || !x || !x_fft || nfields < 1) return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_fft<float>(p, x, x_fft, nfields, s)
+                      : do_fft<double>(p, x, x_fft, nfields, s);
+}
+
+int fsx_ifft(fsx_plan p, const void* x_fft, void* x, int nfields, void* stream) {
+  if (!p || !x || !x_fft || nfields < 1) return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_ifft<float>(p, x_fft, x, nfields, s)
+                      : do_ifft<double>(p, x_fft, x, nfields, s);
+}
+
+int fsx_ns2d_tendency(fsx_plan p, const void* rot_fft, void* out_fft,
+                      double beta, int want_maxima, void* stream) {
+  if (!p || !rot_fft || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 2) return fail(FSX_ERR_ARG, "ns2d needs a 2D plan");
+  if (p->nf_max < 4) return fail(FSX_ERR_ARG, "ns2d needs nf_max >= 4");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_ns2d<float>(p, rot_fft, out_fft, beta, want_maxima, s)
+                      : do_ns2d<double>(p, rot_fft, out_fft, beta, want_maxima, s);
+}
+
+int fsx_ns3d_tendency(fsx_plan p, const void* v_fft, void* out_fft, int has_f,
+                      double f, int want_maxima, void* stream) {
+  if (!p || !v_fft || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3) return fail(FSX_ERR_ARG, "ns3d needs a 3D plan");
+  if (p->nf_max < 6) return fail(FSX_ERR_ARG, "ns3d needs nf_max >= 6");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4
+             ? do_ns3d<float>(p, v_fft, out_fft, has_f, f, want_maxima, s)
+             : do_ns3d<double>(p, v_fft, out_fft, has_f, f, want_maxima, s);
+}
+
+int fsx_rk_stage(fsx_plan p, int scheme, int stage, int nvar, void* S,
+                 void* tmp, void* t1, void* N, int project, int no_vz_kz0,
+                 const double* visc, void* stream) {
+  if (!p || !S || !t1 || !N || !visc) return fail(FSX_ERR_ARG, "bad argument");
+  if (scheme != 4 && scheme != 2) return fail(FSX_ERR_ARG, "scheme must be 2 or 4");
+  if (stage < 0 || stage >= scheme) return fail(FSX_ERR_ARG, "bad stage");
+  if (scheme == 4 && !tmp) return fail(FSX_ERR_ARG, "RK4 needs tmp");
+  if (nvar < 1 || nvar > 4) return fail(FSX_ERR_ARG, "nvar must be 1..4");
+  if (project && (p->dim != 3 || nvar < 3))
+    return fail(FSX_ERR_ARG, "projection needs a 3D plan and 3 velocity fields");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_rk<float>(p, scheme, stage, nvar, S, tmp, t1, N,
+                                     project, no_vz_kz0, visc, s)
+                      : do_rk<double>(p, scheme, stage, nvar, S, tmp, t1, N,
+                                      project, no_vz_kz0, visc, s);
+}
+
+int fsx_project(fsx_plan p, void* a, void* b, void* c, int kind, void* stream) {
+  if (!p || !a || !b || !c || p->dim != 3) return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  const Box box = make_box(p, false);
+  const size_t n = kelems(p);
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_project<float>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (cpx<float>*)a, (cpx<float>*)b, (cpx<float>*)c, kind, box);
+  else
+    hipLaunchKernelGGL(k_project<double>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (cpx<double>*)a, (cpx<double>*)b, (cpx<double>*)c, kind, box);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_dealias(fsx_plan p, void* arr, int nfields, int box_only, void* stream) {
+  if (!p || !arr || nfields < 1) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->dealias) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const Box box = make_box(p, true);
+  const unsigned char* mask = box_only ? nullptr : p->d_mask;
+  const size_t n = kelems(p);
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_dealias<float>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (cpx<float>*)arr, n, nfields, box, mask);
+  else
+    hipLaunchKernelGGL(k_dealias<double>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (cpx<double>*)arr, n, nfields, box, mask);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_set_dt(fsx_plan p, double dt, void* stream) {
+  if (!p) return fail(FSX_ERR_ARG, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  static thread_local double h;
+  h = dt;
+  FSX_HIP(hipMemcpyAsync(p->d_scal + 4, &h, sizeof(double), hipMemcpyHostToDevice, s));
+  FSX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsx_reset_maxima(fsx_plan p, void* stream) {
+  if (!p) return fail(FSX_ERR_ARG, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
+                     reinterpret_cast<unsigned long long*>(p->d_scal), 4, 0ull);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cfl_update(fsx_plan p, int ncomp, double cfl, double dt_max,
+                   void* stream) {
+  if (!p || ncomp < 1 || ncomp > 3) return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  const double dx = p->Lx / p->nx, dy = p->Ly / p->ny,
+               dz = p->dim == 3 ? p->Lz / p->nz : 1.0;
+  hipLaunchKernelGGL(k_cfl_update, dim3(1), dim3(64), 0, s,
+                     reinterpret_cast<const unsigned long long*>(p->d_scal),
+                     ncomp, dx, dy, dz, cfl, dt_max, p->d_scal + 4);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_get_scalars(fsx_plan p, double* host6, void* stream) {
+  if (!p || !host6) return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  double h[6];
+  FSX_HIP(hipMemcpyAsync(h, p->d_scal, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+  FSX_HIP(hipStreamSynchronize(s));
+  unsigned long long bits;
+  for (int i = 0; i < 4; ++i) {
+    std::memcpy(&bits, &h[i], 8);
+    std::memcpy(&host6[i], &bits, 8);
+  }
+  host6[4] = h[4];
+  unsigned int flag;
+  std::memcpy(&flag, &h[5], 4);
+  host6[5] = (double)flag;
+  return 0;
+}
+
+int fsx_clear_nan_flag(fsx_plan p, void* stream) {
+  if (!p) return fail(FSX_ERR_ARG, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
+                     reinterpret_cast<unsigned long long*>(p->d_scal + 5), 1, 0ull);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_sums(fsx_plan p, int kind, const void* f0, const void* f1,
+             const void* f2, const void* f3, double N2, const double* visc,
+             double* host_out6, void* stream) {
+  if (!p || !f0 || !visc || !host_out6) return fail(FSX_ERR_ARG, "bad argument");
+  if (kind != 0 && (!f1 || !f2)) return fail(FSX_ERR_ARG, "missing fields");
+  if (kind == 2 && (!f3 || !(N2 > 0))) return fail(FSX_ERR_ARG, "strat needs b and N");
+  hipStream_t s = (hipStream_t)stream;
+  if (!f1) f1 = f0;
+  if (!f2) f2 = f0;
+  if (!f3) f3 = f0;
+  return p->prec == 4
+             ? do_sums<float>(p, kind, f0, f1, f2, f3, N2, visc, host_out6, s)
+             : do_sums<double>(p, kind, f0, f1, f2, f3, N2, visc, host_out6, s);
+}
+
+int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
+                 int nf, int nbins, double* host_out, void* stream) {
+  if (!p || !f0 || nf < 1 || nf > 3 || nbins < 1 || !host_out)
+    return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->spec_cap < nbins) {
+    if (p->d_spec) (void)hipFree(p->d_spec);
+    p->d_spec = nullptr;
+    FSX_HIP(hipMalloc((void**)&p->d_spec, nbins * sizeof(double)));
+    p->spec_cap = nbins;
+  }
+  FSX_HIP(hipMemsetAsync(p->d_spec, 0, nbins * sizeof(double), s));
+  const size_t n = kelems(p);
+  const Box box = make_box(p, false);
+  if (!f1) f1 = f0;
+  if (!f2) f2 = f0;
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_spectrum<float>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (const cpx<float>*)f0, (const cpx<float>*)f1,
+                       (const cpx<float>*)f2, nf, box, nbins, p->d_spec);
+  else
+    hipLaunchKernelGGL(k_spectrum<double>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (const cpx<double>*)f0, (const cpx<double>*)f1,
+                       (const cpx<double>*)f2, nf, box, nbins, p->d_spec);
+  FSX_HIP(hipGetLastError());
+  FSX_HIP(hipMemcpyAsync(host_out, p->d_spec, nbins * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  FSX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,346 @@
+// Pointwise spectral kernels of the fluidsim_amd engine: integrating-factor
+// Runge-Kutta stages (with the Leray / toroidal / poloidal projection folded
+// in), standalone projection and dealiasing, FP64 spectral sums and the CFL
+// time-step update.
+#pragma once
+
+#include "fsx_fft.h"
+
+namespace fsx {
+
+// Box of kept modes iterated by the pointwise kernels:
+// kz in zm (compact), ky in ym (compact), kx in [0, mx).
+struct Box {
+  int dim, nx, ny, nz, nkx, mx;
+  AxisMap ym, zm;
+  double dkx, dky, dkz;
+  __device__ size_t count() const {
+    return (size_t)(dim == 3 ? zm.nk() : 1) * ym.nk() * mx;
+  }
+  // compact linear index -> full K index and integer wavenumbers
+  __device__ size_t locate(size_t i, int& ikx, int& iky, int& ikz) const {
+    ikx = (int)(i % mx);
+    size_t rest = i / mx;
+    const int jy = (int)(rest % ym.nk());
+    const int jz = (int)(rest / ym.nk());
+    const int iy = ym.full(jy);
+    const int iz = dim == 3 ? zm.full(jz) : 0;
+    iky = ksigned(iy, ny);
+    ikz = dim == 3 ? ksigned(iz, nz) : 0;
+    return ((size_t)iz * ny + iy) * nkx + ikx;
+  }
+};
+
+struct Visc {
+  double nu2, nu4, nu8, num4;
+};
+
+// sigma(k) = nu2 k^2 + nu4 k^4 + nu8 k^8 (+ nu_m4 / k^4, whose k = 0 value
+// is the one of the first kx mode), fluidsim/base/solvers/pseudo_spect.py
+// :134-191.
+__device__ __forceinline__ double freq_diss(const Box& b, const Visc& v,
+                                            int ikx, int iky, int ikz) {
+  const double kx = ikx * b.dkx, ky = iky * b.dky, kz = ikz * b.dkz;
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  double s = 0.0;
+  if (v.nu2 > 0) s = v.nu2 * k2;
+  const double k4 = k2 * k2;
+  if (v.nu4 > 0) s += v.nu4 * k4;
+  if (v.nu8 > 0) s += v.nu8 * (k4 * k4);
+  if (v.num4 != 0) {
+    double k2n = k2;
+    if (ikx == 0 && iky == 0 && ikz == 0) k2n = b.dkx * b.dkx;
+    s += v.num4 / (k2n * k2n);
+  }
+  return s;
+}
+
+// projection kinds
+enum { PROJ_NONE = 0, PROJ_PERPK = 1, PROJ_TOROIDAL = 2, PROJ_POLOIDAL = 3 };
+
+template <typename T>
+__device__ __forceinline__ void project3(cpx<T>& a, cpx<T>& b, cpx<T>& c,
+                                         int kind, int ikx, int iky, int ikz,
+                                         const Box& bx) {
+  const T kx = T(ikx * bx.dkx), ky = T(iky * bx.dky), kz = T(ikz * bx.dkz);
+  if (kind == PROJ_PERPK) {
+    T k2 = kx * kx + ky * ky + kz * kz;
+    if (ikx == 0 && iky == 0 && ikz == 0) k2 = T(1e-14);
+    const T inv = T(1) / k2;
+    const cpx<T> t = scale(scale(a, kx) + scale(b, ky) + scale(c, kz), inv);
+    a = a - scale(t, kx);
+    b = b - scale(t, ky);
+    c = c - scale(t, kz);
+  } else if (kind == PROJ_TOROIDAL || kind == PROJ_POLOIDAL) {
+    const T kh2 = kx * kx + ky * ky;
+    T kh2n = kh2 == T(0) ? T(1e-14) : kh2;
+    T k2n = kh2 + kz * kz;
+    if (k2n == T(0)) k2n = T(1e-14);
+    const T cphi = kx * sqrt(T(1) / kh2n), sphi = ky * sqrt(T(1) / kh2n);
+    if (kind == PROJ_TOROIDAL) {
+      const cpx<T> t = scale(b, cphi) - scale(a, sphi);
+      a = scale(t, -sphi);
+      b = scale(t, cphi);
+      c = zero<T>();
+    } else {
+      const T cth = kz * sqrt(T(1) / k2n), sth = sqrt(kh2 / k2n);
+      const cpx<T> t =
+          scale(a, cth * cphi) + scale(b, cth * sphi) - scale(c, sth);
+      a = scale(t, cth * cphi);
+      b = scale(t, cth * sphi);
+      c = scale(t, -sth);
+    }
+  }
+}
+
+template <typename T>
+struct RKArgs {
+  cpx<T>* S[4];
+  cpx<T>* tmp[4];
+  cpx<T>* t1[4];
+  cpx<T>* Nt[4];  // tendencies (projected in place when project != 0)
+  int nvar;
+  int scheme;  // 4: RK4, 2: RK2
+  int stage;
+  int project;  // projection kind applied to variables 0..2 (3D)
+  int no_vz_kz0;
+  const double* dt;
+  Visc visc;
+  Box box;
+  unsigned int* nanflag;
+};
+
+// One integrating-factor Runge-Kutta stage over the kept modes
+// (fluidsim/base/time_stepping/pseudo_spect.py:51-68, 466-517, 898-984).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
+  const size_t n = a.box.count();
+  const double dt = *a.dt;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    int ikx, iky, ikz;
+    const size_t idx = a.box.locate(i, ikx, iky, ikz);
+    const double sig = freq_diss(a.box, a.visc, ikx, iky, ikz);
+    const T e = T(exp(-dt * sig));
+    const T e2 = T(exp(-dt / 2 * sig));
+    cpx<T> N[4];
+    for (int v = 0; v < a.nvar; ++v) N[v] = a.Nt[v][idx];
+    if (a.project != PROJ_NONE) {
+      project3(N[0], N[1], N[2], a.project, ikx, iky, ikz, a.box);
+      if (a.no_vz_kz0 && ikz == 0) {
+        N[2] = zero<T>();
+        if (a.nvar > 3) N[3] = zero<T>();
+      }
+    }
+    cpx<T> Snew[4];
+    for (int v = 0; v < a.nvar; ++v) {
+      const cpx<T> S = a.S[v][idx];
+      const cpx<T> Nv = N[v];
+      if (a.scheme == 4) {
+        if (a.stage == 0) {
+          a.tmp[v][idx] = scale(S + scale(Nv, T(dt / 6)), e);
+          a.t1[v][idx] = scale(S + scale(Nv, T(dt / 2)), e2);
+        } else if (a.stage == 1) {
+          a.tmp[v][idx] = a.tmp[v][idx] + scale(Nv, T(dt / 3) * e2);
+          a.t1[v][idx] = scale(S, e2) + scale(Nv, T(dt / 2));
+        } else if (a.stage == 2) {
+          a.tmp[v][idx] = a.tmp[v][idx] + scale(Nv, T(dt / 3) * e2);
+          a.t1[v][idx] = scale(S, e) + scale(Nv, T(dt) * e2);
+        } else {
+          Snew[v] = a.tmp[v][idx] + scale(Nv, T(dt / 6));
+        }
+      } else {
+        if (a.stage == 0) {
+          a.t1[v][idx] = scale(S + scale(Nv, T(dt / 2)), e2);
+        } else {
+          Snew[v] = scale(S, e) + scale(Nv, T(dt) * e2);
+        }
+      }
+    }
+    const bool last = (a.scheme == 4 && a.stage == 3) ||
+                      (a.scheme == 2 && a.stage == 1);
+    if (last) {
+      if (a.project != PROJ_NONE) {
+        project3(Snew[0], Snew[1], Snew[2], a.project, ikx, iky, ikz, a.box);
+        if (a.no_vz_kz0 && ikz == 0) {
+          Snew[2] = zero<T>();
+          if (a.nvar > 3) Snew[3] = zero<T>();
+        }
+      }
+      for (int v = 0; v < a.nvar; ++v) a.S[v][idx] = Snew[v];
+      if (isnan(Snew[0].x) || isnan(Snew[0].y)) atomicOr(a.nanflag, 1u);
+    }
+  }
+}
+
+// Standalone projection of 3 fields over the full K array.
+template <typename T>
+__global__ __launch_bounds__(256) void k_project(cpx<T>* a, cpx<T>* b,
+                                                 cpx<T>* c, int kind, Box box) {
+  const size_t n = (size_t)box.nz * box.ny * box.nkx;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ikx = (int)(i % box.nkx);
+    const size_t rest = i / box.nkx;
+    const int iy = (int)(rest % box.ny), iz = (int)(rest / box.ny);
+    cpx<T> va = a[i], vb = b[i], vc = c[i];
+    project3(va, vb, vc, kind, ikx, ksigned(iy, box.ny),
+             box.dim == 3 ? ksigned(iz, box.nz) : 0, box);
+    a[i] = va;
+    b[i] = vb;
+    c[i] = vc;
+  }
+}
+
+// Zero the dealiased modes of nf fields (mask or index box).
+template <typename T>
+__global__ __launch_bounds__(256) void k_dealias(cpx<T>* base, size_t fstride,
+                                                 int nf, Box box,
+                                                 const unsigned char* mask) {
+  const size_t n = (size_t)box.nz * box.ny * box.nkx;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    bool dealiased;
+    if (mask) {
+      dealiased = mask[i] != 0;
+    } else {
+      const int ikx = (int)(i % box.nkx);
+      const size_t rest = i / box.nkx;
+      const int iy = (int)(rest % box.ny), iz = (int)(rest / box.ny);
+      dealiased = ikx >= box.mx || box.ym.compact(iy) < 0 ||
+                  (box.dim == 3 && box.zm.compact(iz) < 0);
+    }
+    if (dealiased)
+      for (int f = 0; f < nf; ++f) base[f * fstride + i] = zero<T>();
+  }
+}
+
+// FP64 Parseval-weighted sums over the full K array.
+//   kind 0 (ns2d, field = rot): E, Z, epsK, epsK_hypo, epsZ, epsZ_hypo
+//   kind 1 (3D, fields = v):    E, epsK, epsK_hypo
+//   kind 2 (strat, v and b):    E_K, E_A (= |b|^2 / (2 N^2)), epsK, epsA
+template <typename T>
+__global__ __launch_bounds__(256) void k_sums(const cpx<T>* f0,
+                                              const cpx<T>* f1,
+                                              const cpx<T>* f2,
+                                              const cpx<T>* f3, int kind,
+                                              double N2, Box box, Visc visc,
+                                              double* out) {
+  const size_t n = (size_t)box.nz * box.ny * box.nkx;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const bool even = (box.nx & 1) == 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ikx = (int)(i % box.nkx);
+    const size_t rest = i / box.nkx;
+    const int iy = (int)(rest % box.ny), iz = (int)(rest / box.ny);
+    const int iky = ksigned(iy, box.ny);
+    const int ikz = box.dim == 3 ? ksigned(iz, box.nz) : 0;
+    const double w =
+        (ikx == 0 || (even && ikx == box.nx / 2)) ? 1.0 : 2.0;
+    const double kx = ikx * box.dkx, ky = iky * box.dky, kz = ikz * box.dkz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    // f_d (small scales) and f_d_hypo separately
+    Visc vs = visc;
+    vs.num4 = 0.0;
+    const double fd = freq_diss(box, vs, ikx, iky, ikz);
+    Visc vh = {0.0, 0.0, 0.0, visc.num4};
+    const double fh = visc.num4 != 0 ? freq_diss(box, vh, ikx, iky, ikz) : 0.0;
+    if (kind == 0) {
+      const cpx<T> r = f0[i];
+      const double a2 = (double)r.x * r.x + (double)r.y * r.y;
+      const double k2n = k2 == 0 ? 1e-14 : k2;
+      const double e = 0.5 * a2 * k2 / (k2n * k2n);
+      const double z = 0.5 * a2;
+      acc[0] += w * e;
+      acc[1] += w * z;
+      acc[2] += w * fd * 2 * e;
+      acc[3] += w * fh * 2 * e;
+      acc[4] += w * fd * 2 * z;
+      acc[5] += w * fh * 2 * z;
+    } else {
+      const cpx<T> a = f0[i], b = f1[i], c = f2[i];
+      const double e =
+          0.5 * ((double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x +
+                 (double)b.y * b.y + (double)c.x * c.x + (double)c.y * c.y);
+      acc[0] += w * e;
+      if (kind == 1) {
+        acc[1] += w * fd * 2 * e;
+        acc[2] += w * fh * 2 * e;
+      } else {
+        const cpx<T> bb = f3[i];
+        const double ea = 0.5 * ((double)bb.x * bb.x + (double)bb.y * bb.y) / N2;
+        acc[1] += w * ea;
+        acc[2] += w * (fd + fh) * 2 * e;
+        acc[3] += w * (fd + fh) * 2 * ea;
+      }
+    }
+  }
+  __shared__ double red[6][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = 0; q < 6; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[q][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double s = 0;
+    for (int w2 = 0; w2 < 4; ++w2) s += red[threadIdx.x][w2];
+    atomicAdd(out + threadIdx.x, s);
+  }
+}
+
+// Shell spectrum E(k_j) = sum over shell j of w |f|^2 / 2 / dk with
+// j = round(|k| / dk), dk = dkx (sum over nf fields).
+template <typename T>
+__global__ __launch_bounds__(256) void k_spectrum(const cpx<T>* f0,
+                                                  const cpx<T>* f1,
+                                                  const cpx<T>* f2, int nf,
+                                                  Box box, int nbins,
+                                                  double* out) {
+  const size_t n = (size_t)box.nz * box.ny * box.nkx;
+  const bool even = (box.nx & 1) == 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ikx = (int)(i % box.nkx);
+    const size_t rest = i / box.nkx;
+    const int iy = (int)(rest % box.ny), iz = (int)(rest / box.ny);
+    const double kx = ikx * box.dkx, ky = ksigned(iy, box.ny) * box.dky;
+    const double kz = box.dim == 3 ? ksigned(iz, box.nz) * box.dkz : 0.0;
+    const double kk = sqrt(kx * kx + ky * ky + kz * kz);
+    const int j = (int)rint(kk / box.dkx);
+    if (j >= nbins) continue;
+    const double w = (ikx == 0 || (even && ikx == box.nx / 2)) ? 1.0 : 2.0;
+    double e = 0;
+    const cpx<T>* fs[3] = {f0, f1, f2};
+    for (int q = 0; q < nf; ++q) {
+      const cpx<T> a = fs[q][i];
+      e += 0.5 * ((double)a.x * a.x + (double)a.y * a.y);
+    }
+    atomicAdd(out + j, w * e / box.dkx);
+  }
+}
+
+// CFL update (fluidsim/base/time_stepping/base.py:320-366): tmp =
+// sum_i max|u_i| / dx_i, dt_CFL = CFL / tmp, new = min(dt_CFL, dt_max),
+// kept unless the relative change exceeds 2 %.
+__global__ void k_cfl_update(const unsigned long long* maxima, int ncomp,
+                             double dx, double dy, double dz, double cfl,
+                             double dt_max, double* dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double d[3] = {dx, dy, dz};
+  double tmp = 0;
+  for (int c = 0; c < ncomp; ++c)
+    tmp += __longlong_as_double((long long)maxima[c]) / d[c];
+  const double dtc = tmp > 0 ? cfl / tmp : dt_max;
+  const double nw = fmin(dtc, dt_max);
+  if (fabs(*dt - nw) / nw > 0.02) *dt = nw;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace fsx

@@ -1,0 +1,152 @@
+/* C-ABI of the fluidsim_amd pseudo-spectral engine (MI355X / gfx950).
+ *
+ * One plan per operator (grid, box, precision, dealiasing mask).  Device
+ * buffers are plain pointers to HBM allocated by the caller (the Python host
+ * passes torch tensors' data_ptr()); complex arrays are interleaved
+ * (re, im) of the plan precision, in the sequential fluidfft layout:
+ *   2D  X (ny, nx) real       <->  K (ny, nx/2+1) complex
+ *   3D  X (nz, ny, nx) real   <->  K (nz, ny, nx/2+1) complex
+ * Several fields are contiguous, one array after the other.
+ * Streams are hipStream_t passed as void* (NULL = default stream).
+ * Every function returns 0 on success or a negative FSX_ERR_* code;
+ * fsx_last_error() copies the message of the last failure of the thread.
+ * Calls are stream-ordered; only fsx_get_scalars, fsx_sums, fsx_spectrum and
+ * fsx_set_dt synchronise the stream.  A plan must not be used from two host
+ * threads at once.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the fluidsim source tree).
+ */
+#ifndef FLUIDSIM_AMD_H
+#define FLUIDSIM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSX_ABI_VERSION 1
+
+#define FSX_OK 0
+#define FSX_ERR_ARG (-1)
+#define FSX_ERR_HIP (-2)
+
+typedef struct fsx_plan_s* fsx_plan;
+
+/* ABI version (FSX_ABI_VERSION). */
+int fsx_version(void);
+
+/* Message of the last failure on this thread (NUL-terminated, truncated). */
+int fsx_last_error(char* buf, int len);
+
+/* Create an operator plan.
+ * Replaces the construction of OperatorsPseudoSpectral2D / 3D
+ * (fluidsim/operators/operators2d.py:115-193,
+ *  fluidsim/operators/operators3d.py:207-278): grid, box, FFT plans and the
+ * dealiasing mask.  where_dealiased is the host uint8 mask of shape K
+ * (fluidfft's where_dealiased, possibly modified by truncation_shape,
+ * NO_SHEAR_MODES, NO_KY0) or NULL for no dealiasing.  precision is 4 or 8
+ * (bytes per real).  nf_max is the number of fields the workspace holds
+ * (4 for ns2d, 6 for ns3d, 7 for ns3d.strat).  All device memory of the plan
+ * is allocated here. */
+int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
+                    double Ly, double Lz, int precision,
+                    const unsigned char* where_dealiased, int nf_max);
+int fsx_plan_destroy(fsx_plan p);
+
+/* info[0..11] = dim, precision, nx, ny, nz, nkx, kept kx count, kept ky
+ * (lo, hi), kept kz (lo, hi), mask-mode flag. */
+int fsx_plan_info(fsx_plan p, int* info, int n);
+
+/* Forward real transform divided by the number of points, nfields fields.
+ * Replaces fft2 / fft3d / fft_as_arg
+ * (fluidfft, called at fluidsim/solvers/ns2d/solver.py:181,
+ *  fluidsim/solvers/ns3d/solver.py:237-239). */
+int fsx_fft(fsx_plan p, const void* x, void* x_fft, int nfields, void* stream);
+
+/* Unnormalised inverse real transform; x_fft is not modified.  The
+ * imaginary parts of the kx = 0 and kx = nx/2 columns are ignored after the
+ * y/z passes (numpy/pocketfft c2r semantics).
+ * Replaces ifft2 / ifft3d / ifft_as_arg / ifft_as_arg_destroy
+ * (fluidsim/solvers/ns2d/solver.py:161-171,
+ *  fluidsim/solvers/ns3d/solver.py:210-220). */
+int fsx_ifft(fsx_plan p, const void* x_fft, void* x, int nfields, void* stream);
+
+/* ns2d nonlinear tendency N = dealias(fft(-u d_x rot - v (d_y rot + beta))).
+ * Replaces Simul.tendencies_nonlin + compute_Frot
+ * (fluidsim/solvers/ns2d/solver.py:33-39, 111-194).  Only non-dealiased
+ * modes of out_fft are written (dealiased ones inside the kept box as 0);
+ * the caller keeps out_fft zero outside the box (fsx_dealias with
+ * box_only = 1).  want_maxima stores max|ux|, max|uy| for fsx_cfl_update. */
+int fsx_ns2d_tendency(fsx_plan p, const void* rot_fft, void* out_fft,
+                      double beta, int want_maxima, void* stream);
+
+/* ns3d nonlinear tendency fft(v x omega) with omega = i k x v (+ f at k = 0),
+ * dealiased, NOT projected (fsx_project or the projection folded into
+ * fsx_rk_stage does it).  v_fft and out_fft hold 3 fields.
+ * Replaces Simul.tendencies_nonlin (fluidsim/solvers/ns3d/solver.py:180-253)
+ * up to project_state_spect. */
+int fsx_ns3d_tendency(fsx_plan p, const void* v_fft, void* out_fft, int has_f,
+                      double f, int want_maxima, void* stream);
+
+/* One integrating-factor Runge-Kutta stage over the kept modes, with
+ * E = exp(-dt sigma(k)), E2 = exp(-dt/2 sigma(k)) computed in FP64 from the
+ * device time step (fsx_set_dt / fsx_cfl_update), sigma from
+ * visc = {nu_2, nu_4, nu_8, nu_m4}.  scheme 4 (stages 0..3) or 2 (0..1).
+ * project (3D): 0 none, 1 Leray (project_perpk3d), 2 toroidal, 3 poloidal,
+ * applied to N and, at the last stage, to the new state.
+ * Replaces _time_step_RK4 / _time_step_RK2 with ExactLinearCoefs
+ * (fluidsim/base/time_stepping/pseudo_spect.py:103-152, 466-517, 798-984)
+ * and the projection of one_time_step_computation
+ * (fluidsim/solvers/ns3d/time_stepping.py:8-20).  Sets the NaN flag read by
+ * fsx_get_scalars (pseudo_spect.py:241-243). */
+int fsx_rk_stage(fsx_plan p, int scheme, int stage, int nvar, void* S,
+                 void* tmp, void* t1, void* N, int project, int no_vz_kz0,
+                 const double* visc, void* stream);
+
+/* In-place projection of (a, b, c) over all modes: kind 1 project_perpk3d,
+ * 2 project_toroidal, 3 project_poloidal
+ * (fluidsim/operators/operators3d.py:787-937, fluidfft project_perpk3d). */
+int fsx_project(fsx_plan p, void* a, void* b, void* c, int kind, void* stream);
+
+/* Zero dealiased modes of nfields fields (box_only: only outside the
+ * bounding box of kept modes).
+ * Replaces dealiasing (fluidsim/operators/operators2d.py:200-220,
+ * fluidsim/operators/operators3d.py:38-71, 336-342). */
+int fsx_dealias(fsx_plan p, void* arr, int nfields, int box_only, void* stream);
+
+/* Device time step used by fsx_rk_stage (synchronous). */
+int fsx_set_dt(fsx_plan p, double dt, void* stream);
+
+/* Zero the CFL maxima slots before a tendency with want_maxima. */
+int fsx_reset_maxima(fsx_plan p, void* stream);
+
+/* dt update from the maxima on the device
+ * (fluidsim/base/time_stepping/base.py:320-366): ncomp velocity components,
+ * CFL coefficient, deltat_max, 2 % hysteresis. */
+int fsx_cfl_update(fsx_plan p, int ncomp, double cfl, double dt_max,
+                   void* stream);
+
+/* host6 = {max|u_0|, max|u_1|, max|u_2|, (unused), dt, nan flag} (sync). */
+int fsx_get_scalars(fsx_plan p, double* host6, void* stream);
+int fsx_clear_nan_flag(fsx_plan p, void* stream);
+
+/* FP64 Parseval-weighted sums (sync), host_out6:
+ * kind 0 (ns2d, f0 = rot): E, Z, epsK, epsK_hypo, epsZ, epsZ_hypo
+ *   (fluidsim/solvers/ns2d/output/spatial_means.py:23-38);
+ * kind 1 (f0..f2 = v): E, epsK, epsK_hypo
+ *   (fluidsim/solvers/ns3d/output/spatial_means.py:23-73);
+ * kind 2 (f0..f2 = v, f3 = b, N2 = N^2): E_K, E_A, epsK, epsA. */
+int fsx_sums(fsx_plan p, int kind, const void* f0, const void* f1,
+             const void* f2, const void* f3, double N2, const double* visc,
+             double* host_out6, void* stream);
+
+/* Shell spectrum of nf fields: out[j] = sum over round(|k|/dkx) = j of
+ * w |f|^2 / 2 / dkx, j < nbins (sync). */
+int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
+                 int nf, int nbins, double* host_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDSIM_AMD_H */
