@@ -1,0 +1,226 @@
+#!/usr/bin/env python
+"""Benchmark: RK4 steps/s of the ns3d solver on one MI355X.
+
+Workload (default): ns3d, periodic (2 pi)^3, 512^3, FP32, RK4, order-8
+hyperviscosity with k_max * eta = 1, seeded isotropic solenoidal initial
+velocity with E(k) ~ k^4 exp(-2 (k/4)^2) normalised to unit energy, fixed dt
+taken from the initial CFL condition at 0.5.  A step is one full RK4 step
+(4 fused tendency evaluations + 4 Runge-Kutta stages) with the state
+resident in HBM.
+
+Prints ONE JSON line with the metric, a ``roofline`` object for the
+dominant kernel (algorithmic bytes per launch / average launch duration,
+measured with HIP events over the timed region), the step-level roofline of
+the algorithmic bytes B_alg = T (b_X + b_K) + R b_K per RK4 step
+(T = 36 real transforms, R = 51 state streams for ns3d), and a
+``cpu_baseline`` from the FP64 numpy oracle timed on the host cores.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "RK4 steps/s + %HBM roofline; closure-LES vs DNS speedup at equal error"
+PEAK_HBM_GBPS = 8000.0
+KINDS = ["z-pass inverse", "y-pass inverse", "x row pass", "y-pass forward",
+         "z-pass forward", "RK stage", "other", "unused"]
+
+
+def b_alg_ns3d(n, prec):
+    bX = prec * n**3
+    bK = 2 * prec * n * n * (n // 2 + 1)
+    return 36 * (bX + bK) + 51 * bK
+
+
+def make_sim(n, precision, seed=0):
+    from fluidsim_amd.solvers.ns3d.solver import Simul
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = params.oper.nz = n
+    params.oper.precision = precision
+    params.nu_8 = nu8_for_kmax_eta(n)
+    params.time_stepping.USE_CFL = True
+    params.time_stepping.cfl_coef = 0.5
+    params.time_stepping.USE_T_END = False
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    sim = Simul(params)
+    cdt = np.complex64 if precision == "float32" else np.complex128
+    vv = ns3d_spectrum_vv_fft(n, seed=seed, dtype=cdt)
+    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+    del vv
+    # fixed dt from the initial CFL condition
+    sim.time_stepping.compute_time_increment_CLF()
+    params.time_stepping.USE_CFL = False
+    return sim
+
+
+def cpu_baseline(n_sample, n_metric, steps=1):
+    """FP64 oracle (numpy + scipy.fft on all host cores of this process)."""
+    from oracle import operators as O
+    from oracle.solvers import NS3D
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    O.set_workers(cores)
+    op = O.Oper3D(n_sample, n_sample, n_sample)
+    sim = NS3D(op, nu_8=nu8_for_kmax_eta(n_sample), USE_CFL=True, cfl_coef=0.5)
+    vv = ns3d_spectrum_vv_fft(n_sample, seed=0)
+    sim.state_spect[:] = np.array(vv)
+    del vv
+    sim.compute_time_increment_CLF()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.one_time_step_computation()
+    dt = (time.perf_counter() - t0) / steps
+    N1 = n_sample**3
+    N2 = n_metric**3
+    scale = (N2 * math.log2(N2)) / (N1 * math.log2(N1))
+    return {
+        "value": 1.0 / (dt * scale),
+        "unit": "RK4 steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (
+            f"{steps} RK4 step of the same ns3d workload at {n_sample}^3 FP64 "
+            f"({dt:.2f} s/step with scipy.fft workers={cores}, numpy elementwise "
+            f"single-threaded), scaled by N log N (x{scale:.2f}) to {n_metric}^3"
+        ),
+    }
+
+
+def read_traffic(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--precision", default="float32", choices=["float32", "float64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(
+            "bench.py measures one MI355X: this engine has no multi-GPU path in "
+            "this build (slab/pencil sharding over RCCL is design-only). "
+            "Run with --gpus 1.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import torch
+    from fluidsim_amd import _lib
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n, prec = args.n, (4 if args.precision == "float32" else 8)
+    print(f"[bench] building ns3d {n}^3 {args.precision}", file=sys.stderr, flush=True)
+    sim = make_sim(n, args.precision)
+    ts = sim.time_stepping
+    plan = sim.oper.plan.handle
+    print(f"[bench] dt = {ts.deltat:.6g}; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ts.one_time_step_computation()
+    torch.cuda.synchronize()
+
+    _lib.call("fsx_timing_enable", plan, 1)
+    ms8 = (_lib.ctypes.c_double * 8)()
+    by8 = (_lib.ctypes.c_double * 8)()
+    cn8 = (_lib.ctypes.c_longlong * 8)()
+    _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts.one_time_step_computation()
+        ts.t += ts.deltat
+        ts.it += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+    _lib.call("fsx_timing_enable", plan, 0)
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    steps_per_s = args.steps / elapsed
+    kernels = {}
+    for k in range(7):
+        if cn8[k]:
+            kernels[KINDS[k]] = {
+                "avg_ms": ms8[k] / cn8[k],
+                "launches_per_step": cn8[k] / args.steps,
+                "share_of_step": ms8[k] / (elapsed * 1e3),
+                "alg_GBps": by8[k] / (ms8[k] * 1e-3) / 1e9,
+                "alg_MB_per_launch": by8[k] / cn8[k] / 1e6,
+            }
+    dom = max(range(7), key=lambda k: ms8[k])
+    achieved = by8[dom] / (ms8[dom] * 1e-3) / 1e9
+    balg = b_alg_ns3d(n, prec)
+    traffic = read_traffic(f"ns3d_{n}_{args.precision}:{KINDS[dom]}")
+
+    result = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == 4 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (
+                f"ns3d {n}^3 {args.precision} RK4, nu_8 with k_max*eta=1, "
+                "E(k)~k^4 exp(-2(k/4)^2) unit-energy solenoidal init (seed 0), "
+                "fixed dt from initial CFL 0.5, 2/3 dealiasing"
+            ),
+            "grid": [n, n, n],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": KINDS[dom],
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": traffic,
+        },
+        "step_roofline": {
+            "B_alg_GB_per_step": balg / 1e9,
+            "achieved_GBps": balg * steps_per_s / 1e9,
+            "frac": balg * steps_per_s / 1e9 / PEAK_HBM_GBPS,
+        },
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        del sim
+        torch.cuda.empty_cache()
+        result["cpu_baseline"] = cpu_baseline(args.cpu_n, n)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

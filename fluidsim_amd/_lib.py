@@ -56,6 +56,11 @@ SIGNATURES = {
         ctypes.POINTER(c_double), P,
     ],
     "fsx_spectrum": [P, P, P, P, c_int, c_int, ctypes.POINTER(c_double), P],
+    "fsx_timing_enable": [P, c_int],
+    "fsx_timing_read": [
+        P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
+        ctypes.POINTER(ctypes.c_longlong), c_int,
+    ],
 }
 
 

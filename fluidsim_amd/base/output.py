@@ -1,0 +1,143 @@
+"""Output (fluidsim/base/output/base.py:60-521, subset).
+
+Periodic printing (print_stdout, periods_print.print_stdout) and spatial
+means (periods_save.spatial_means) computed with FP64 sums on the device;
+energy spectra on request.  Files (stdout.txt, spatial_means.txt,
+params.json) are written to path_run when params.output.HAS_TO_SAVE.
+Physical-field state files and plots are not implemented yet.
+"""
+
+import json
+import os
+from datetime import datetime
+
+import numpy as np
+
+from .. import _lib
+
+
+class OutputBasePseudoSpectral:
+    @staticmethod
+    def _complete_params_with_default(params, info_solver=None):
+        params._set_child(
+            "output",
+            attribs={
+                "HAS_TO_SAVE": True,
+                "sub_directory": "",
+                "ONLINE_PLOT_OK": False,
+            },
+        )
+        params.output._set_child(
+            "periods_save",
+            attribs={"phys_fields": 0, "spatial_means": 0, "spectra": 0},
+        )
+        params.output._set_child("periods_print", attribs={"print_stdout": 1.0})
+        params.output._set_child("periods_plot", attribs={"phys_fields": 0})
+        params.output._set_child("phys_fields", attribs={"field_to_plot": ""})
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.params = sim.params
+        self.oper = sim.oper
+        p = self.params
+        name = "_".join(
+            s for s in (
+                sim.info_solver.short_name,
+                p.short_name_type_run,
+                self.oper.produce_str_describing_oper(),
+                datetime.now().strftime("%Y-%m-%d_%H-%M-%S"),
+            ) if s
+        )
+        self.name_run = name
+        self._has_to_save = bool(p.output.HAS_TO_SAVE)
+        base = os.environ.get("FLUIDSIM_PATH", os.path.expanduser("~/Sim_data"))
+        self.path_run = os.path.join(base, p.output.sub_directory, name)
+        self._t_last = {}
+        self.spatial_means_results = []
+        self._end_of_simul = False
+
+    # ----------------------------------------------------------------
+    def post_init(self):
+        if self._has_to_save:
+            os.makedirs(self.path_run, exist_ok=True)
+            with open(os.path.join(self.path_run, "params.json"), "w") as f:
+                json.dump(self.params._make_dict_tree(), f, indent=1, default=str)
+        self.print_stdout(
+            f"*************************************\nfluidsim_amd solver "
+            f"{self.sim.info_solver.short_name}\nsim.name_run: {self.name_run}"
+        )
+
+    def print_stdout(self, to_print, end="\n"):
+        print(to_print, end=end, flush=True)
+        if self._has_to_save:
+            with open(os.path.join(self.path_run, "stdout.txt"), "a") as f:
+                f.write(to_print + end)
+
+    def init_with_initialized_state(self):
+        """base.py:263-300: first CFL, then outputs at t = 0."""
+        if getattr(self, "_has_been_initialized_with_state", False):
+            return
+        self._has_been_initialized_with_state = True
+        if self.params.time_stepping.USE_CFL:
+            self.sim.time_stepping.compute_time_increment_CLF()
+        self._do_outputs(force=True)
+
+    def one_time_step(self):
+        self._do_outputs(force=False)
+
+    def _due(self, key, period, t, force):
+        if not period:
+            return False
+        last = self._t_last.get(key)
+        if force or last is None or t - last >= period:
+            self._t_last[key] = t
+            return True
+        return False
+
+    def _do_outputs(self, force):
+        ts = self.sim.time_stepping
+        t = ts.t
+        p = self.params.output
+        if self._due("print", p.periods_print.print_stdout, t, force):
+            self.print_stdout(
+                f"it = {ts.it:6d} ; t = {t:12.6g} ; deltat  = {ts.deltat:10.5g}"
+            )
+        if self._due("spatial_means", p.periods_save.spatial_means, t, force):
+            self.save_spatial_means()
+
+    def save_spatial_means(self):
+        means = self.compute_spatial_means()
+        means["t"] = self.sim.time_stepping.t
+        self.spatial_means_results.append(means)
+        if self._has_to_save:
+            with open(os.path.join(self.path_run, "spatial_means.txt"), "a") as f:
+                f.write(json.dumps(means) + "\n")
+        return means
+
+    def load_spatial_means(self):
+        keys = self.spatial_means_results[0].keys() if self.spatial_means_results else []
+        return {k: np.array([m[k] for m in self.spatial_means_results]) for k in keys}
+
+    def end_of_simul(self, total_time=None):
+        self._end_of_simul = True
+
+    # ---------------------------------------------------------------- sums
+    def _sums(self, kind, fields, N2=0.0):
+        out = (_lib.ctypes.c_double * 6)()
+        f = [(_lib.dptr(a) if a is not None else None) for a in fields]
+        f += [None] * (4 - len(f))
+        _lib.call("fsx_sums", self.oper.plan.handle, kind, f[0], f[1], f[2], f[3],
+                  float(N2), _lib.doubles(self.sim.visc_coefs()), out,
+                  _lib.stream_ptr())
+        return list(out)
+
+    def compute_spectrum(self, fields, nbins=None):
+        """Shell spectrum sum_{round(|k|/dk)=j} w |f|^2/2 / dk (FP64)."""
+        if nbins is None:
+            nbins = self.oper.nx // 2 + 1
+        out = (_lib.ctypes.c_double * nbins)()
+        f = [_lib.dptr(a) for a in fields] + [None] * (3 - len(fields))
+        _lib.call("fsx_spectrum", self.oper.plan.handle, f[0], f[1], f[2],
+                  len(fields), nbins, out, _lib.stream_ptr())
+        khE = self.oper.deltakx * np.arange(nbins)
+        return khE, np.array(out)

@@ -46,7 +46,23 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 }  // namespace
 
+// Live per-kernel timing: HIP events recorded around each pass launch on the
+// stream it runs on, with the algorithmic bytes of that launch (compulsory
+// HBM reads + writes of its inputs and outputs).
+struct fsx_timing {
+  static constexpr int kKinds = 8;
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // pairs
+  std::vector<int> kind;
+  std::vector<double> bytes;
+  int used = 0;
+  double ms[kKinds] = {0};
+  double by[kKinds] = {0};
+  long long cnt[kKinds] = {0};
+};
+
 struct fsx_plan_s {
+  fsx_timing* tm;
   int dim, prec;
   int nx, ny, nz, nkx;
   double Lx, Ly, Lz, dkx, dky, dkz;
@@ -67,6 +83,64 @@ struct fsx_plan_s {
 };
 
 namespace {
+
+// kinds of launches reported by fsx_timing_read
+enum { K_ZINV = 0, K_YINV = 1, K_XPASS = 2, K_YFWD = 3, K_ZFWD = 4, K_RK = 5,
+       K_OTHER = 6 };
+
+thread_local fsx_plan_s* g_plan = nullptr;  // plan of the running API call
+thread_local int g_kind = K_OTHER;
+thread_local double g_bytes = 0;
+
+struct PlanScope {
+  explicit PlanScope(fsx_plan_s* p) { g_plan = p; }
+  ~PlanScope() { g_plan = nullptr; }
+};
+
+void note(int kind, double bytes) {
+  g_kind = kind;
+  g_bytes = bytes;
+}
+
+int tm_flush(fsx_plan_s* p);
+
+int tm_pre(hipStream_t s) {
+  fsx_plan_s* p = g_plan;
+  if (!p || !p->tm || !p->tm->on) return 0;
+  fsx_timing* t = p->tm;
+  if ((size_t)(2 * t->used + 2) > t->ev.size()) {
+    int rc = tm_flush(p);
+    if (rc) return rc;
+  }
+  FSX_HIP(hipEventRecord(t->ev[2 * t->used], s));
+  return 0;
+}
+
+int tm_post(hipStream_t s) {
+  fsx_plan_s* p = g_plan;
+  if (!p || !p->tm || !p->tm->on) return 0;
+  fsx_timing* t = p->tm;
+  FSX_HIP(hipEventRecord(t->ev[2 * t->used + 1], s));
+  t->kind[t->used] = g_kind;
+  t->bytes[t->used] = g_bytes;
+  ++t->used;
+  return 0;
+}
+
+int tm_flush(fsx_plan_s* p) {
+  fsx_timing* t = p->tm;
+  for (int i = 0; i < t->used; ++i) {
+    FSX_HIP(hipEventSynchronize(t->ev[2 * i + 1]));
+    float ms = 0;
+    FSX_HIP(hipEventElapsedTime(&ms, t->ev[2 * i], t->ev[2 * i + 1]));
+    const int k = t->kind[i];
+    t->ms[k] += ms;
+    t->by[k] += t->bytes[i];
+    t->cnt[k] += 1;
+  }
+  t->used = 0;
+  return 0;
+}
 
 constexpr int kEPT32 = 32;  // 8192 complex64 per tile  = 64 KiB of LDS
 constexpr int kEPT64 = 16;  // 4096 complex128 per tile = 64 KiB of LDS
@@ -121,9 +195,11 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
                                 (int)lds));
     attr = true;
   }
+  int rc = tm_pre(s);
+  if (rc) return rc;
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, s, op);
   FSX_HIP(hipGetLastError());
-  return 0;
+  return tm_post(s);
 }
 
 template <typename T, class Op>
@@ -159,9 +235,11 @@ int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
                                 (int)lds));
     attr = true;
   }
+  int rc = tm_pre(s);
+  if (rc) return rc;
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, s, op);
   FSX_HIP(hipGetLastError());
-  return 0;
+  return tm_post(s);
 }
 
 Box make_box(const fsx_plan_s* p, bool pruned) {
@@ -216,6 +294,7 @@ int inverse_to_w2(fsx_plan_s* p, const cpx<T>* const* kin, int nf, bool pruned,
     op.nkx = p->nkx;
     op.mx = mx;
     op.ym = ym;
+    note(K_YINV, (double)nf * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
     return launch_strided<T, +1>(op, p->ny, 1, mx, nf, p->tw[1], s);
   }
   {
@@ -229,6 +308,7 @@ int inverse_to_w2(fsx_plan_s* p, const cpx<T>* const* kin, int nf, bool pruned,
     op.mx = mx;
     op.ym = ym;
     op.zm = zm;
+    note(K_ZINV, (double)nf * ym.nk() * mx * (zm.nk() + p->nz) * sizeof(cpx<T>));
     int rc = launch_strided<T, +1>(op, p->nz, ym.nk(), mx, nf, p->tw[2], s);
     if (rc) return rc;
   }
@@ -240,6 +320,7 @@ int inverse_to_w2(fsx_plan_s* p, const cpx<T>* const* kin, int nf, bool pruned,
   op.ny = p->ny;
   op.mx = mx;
   op.ym = ym;
+  note(K_YINV, (double)nf * p->nz * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
   return launch_strided<T, +1>(op, p->ny, p->nz, mx, nf, p->tw[1], s);
 }
 
@@ -261,6 +342,7 @@ int forward_from_w2(fsx_plan_s* p, cpx<T>* const* kout, int nf, bool pruned,
     op.mx = mx;
     op.ym = ym;
     op.mask = mask;
+    note(K_YFWD, (double)nf * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
     return launch_strided<T, -1>(op, p->ny, 1, mx, nf, p->tw[1], s);
   }
   {
@@ -272,6 +354,7 @@ int forward_from_w2(fsx_plan_s* p, cpx<T>* const* kout, int nf, bool pruned,
     op.ny = p->ny;
     op.mx = mx;
     op.ym = ym;
+    note(K_YFWD, (double)nf * p->nz * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
     int rc = launch_strided<T, -1>(op, p->ny, p->nz, mx, nf, p->tw[1], s);
     if (rc) return rc;
   }
@@ -286,6 +369,7 @@ int forward_from_w2(fsx_plan_s* p, cpx<T>* const* kout, int nf, bool pruned,
   op.ym = ym;
   op.zm = zm;
   op.mask = mask;
+  note(K_ZFWD, (double)nf * ym.nk() * mx * (p->nz + zm.nk()) * sizeof(cpx<T>));
   return launch_strided<T, -1>(op, p->nz, ym.nk(), mx, nf, p->tw[2], s);
 }
 
@@ -304,6 +388,7 @@ int do_ifft(fsx_plan_s* p, const void* kin, void* xout, int nfields,
     OpXC2R<T> op;
     op.in[0] = W2<T>(p, 0, false);
     op.outr[0] = x;
+    note(K_XPASS, (double)nrows(p) * (p->mxF * sizeof(cpx<T>) + p->nx * sizeof(T)));
     rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, p->mxF, T(1),
                          p->tw[0], s);
     if (rc) return rc;
@@ -321,6 +406,7 @@ int do_fft(fsx_plan_s* p, const void* xin, void* kout, int nfields,
     OpXR2C<T> op;
     op.in[0] = x;
     op.out[0] = W2<T>(p, 0, false);
+    note(K_XPASS, (double)nrows(p) * (p->mxF * sizeof(cpx<T>) + p->nx * sizeof(T)));
     int rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, p->mxF,
                              T(1.0 / npts), p->tw[0], s);
     if (rc) return rc;
@@ -342,6 +428,7 @@ int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
   op.ym = p->ymP;
   op.dkx = (T)p->dkx;
   op.dky = (T)p->dky;
+  note(K_YINV, (double)p->mxP * (p->ymP.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
   int rc = launch_strided<T, +1>(op, p->ny, 1, p->mxP, 4, p->tw[1], s);
   if (rc) return rc;
   OpXNS2D<T> ox;
@@ -351,6 +438,7 @@ int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
   ox.beta = (T)beta;
   const double npts = (double)p->nx * p->ny;
+  note(K_XPASS, 5.0 * p->ny * p->mxP * sizeof(cpx<T>));
   rc = launch_xpass<T>(ox, p->nx, p->ny, 4, 1, p->mxP, p->mxP, T(1.0 / npts),
                        p->tw[0], s);
   if (rc) return rc;
@@ -377,6 +465,8 @@ int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
   op.dkz = (T)p->dkz;
   op.has_f = has_f;
   op.fcor = (T)fcor;
+  note(K_ZINV, (double)p->ymP.nk() * p->mxP * (3.0 * p->zmP.nk() + 6.0 * p->nz) *
+                   sizeof(cpx<T>));
   int rc = launch_strided<T, +1>(op, p->nz, p->ymP.nk(), p->mxP, 6, p->tw[2], s);
   if (rc) return rc;
   OpYInv3<T> oy;
@@ -387,6 +477,7 @@ int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
   oy.ny = p->ny;
   oy.mx = p->mxP;
   oy.ym = p->ymP;
+  note(K_YINV, 6.0 * p->nz * p->mxP * (p->ymP.nk() + p->ny) * sizeof(cpx<T>));
   rc = launch_strided<T, +1>(oy, p->ny, p->nz, p->mxP, 6, p->tw[1], s);
   if (rc) return rc;
   OpXNS3D<T> ox;
@@ -395,6 +486,7 @@ int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
   ox.maxima =
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
   const double npts = (double)p->nx * p->ny * p->nz;
+  note(K_XPASS, 9.0 * nrows(p) * p->mxP * sizeof(cpx<T>));
   rc = launch_xpass<T>(ox, p->nx, nrows(p), 6, 3, p->mxP, p->mxP,
                        T(1.0 / npts), p->tw[0], s);
   if (rc) return rc;
@@ -428,9 +520,14 @@ int do_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S, void* tmp,
   a.box = make_box(p, true);
   a.nanflag = reinterpret_cast<unsigned int*>(p->d_scal + 5);
   const size_t n = (size_t)(p->dim == 3 ? p->zmP.nk() : 1) * p->ymP.nk() * p->mxP;
+  const int nstream = scheme == 4 ? (stage == 0 ? 4 : (stage == 3 ? 3 : 5))
+                                  : (stage == 0 ? 3 : 3);
+  note(K_RK, (double)n * nvar * nstream * 2 * sizeof(T));
+  int rc = tm_pre(s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_rk_stage<T>, dim3(grid_for(n)), dim3(256), 0, s, a);
   FSX_HIP(hipGetLastError());
-  return 0;
+  return tm_post(s);
 }
 
 template <typename T>
@@ -582,6 +679,10 @@ int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
 
 int fsx_plan_destroy(fsx_plan p) {
   if (!p) return 0;
+  if (p->tm) {
+    for (hipEvent_t e : p->tm->ev) (void)hipEventDestroy(e);
+    delete p->tm;
+  }
   for (int a = 0; a < 3; ++a)
     if (p->tw[a]) (void)hipFree(p->tw[a]);
   if (p->w1) (void)hipFree(p->w1);
@@ -603,6 +704,7 @@ int fsx_plan_info(fsx_plan p, int* info, int n) {
 }
 
 int fsx_fft(fsx_plan p, const void* x, void* x_fft, int nfields, void* stream) {
+  PlanScope scope_(p);
   if (!p || !x || !x_fft || nfields < 1) return fail(FSX_ERR_ARG, "bad argument");
   hipStream_t s = (hipStream_t)stream;
   return p->prec == 4 ? do_fft<float>(p, x, x_fft, nfields, s)
@@ -610,6 +712,7 @@ int fsx_fft(fsx_plan p, const void* x, void* x_fft, int nfields, void* stream) {
 }
 
 int fsx_ifft(fsx_plan p, const void* x_fft, void* x, int nfields, void* stream) {
+  PlanScope scope_(p);
   if (!p || !x || !x_fft || nfields < 1) return fail(FSX_ERR_ARG, "bad argument");
   hipStream_t s = (hipStream_t)stream;
   return p->prec == 4 ? do_ifft<float>(p, x_fft, x, nfields, s)
@@ -618,6 +721,7 @@ int fsx_ifft(fsx_plan p, const void* x_fft, void* x, int nfields, void* stream) 
 
 int fsx_ns2d_tendency(fsx_plan p, const void* rot_fft, void* out_fft,
                       double beta, int want_maxima, void* stream) {
+  PlanScope scope_(p);
   if (!p || !rot_fft || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
   if (p->dim != 2) return fail(FSX_ERR_ARG, "ns2d needs a 2D plan");
   if (p->nf_max < 4) return fail(FSX_ERR_ARG, "ns2d needs nf_max >= 4");
@@ -628,6 +732,7 @@ int fsx_ns2d_tendency(fsx_plan p, const void* rot_fft, void* out_fft,
 
 int fsx_ns3d_tendency(fsx_plan p, const void* v_fft, void* out_fft, int has_f,
                       double f, int want_maxima, void* stream) {
+  PlanScope scope_(p);
   if (!p || !v_fft || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
   if (p->dim != 3) return fail(FSX_ERR_ARG, "ns3d needs a 3D plan");
   if (p->nf_max < 6) return fail(FSX_ERR_ARG, "ns3d needs nf_max >= 6");
@@ -640,6 +745,7 @@ int fsx_ns3d_tendency(fsx_plan p, const void* v_fft, void* out_fft, int has_f,
 int fsx_rk_stage(fsx_plan p, int scheme, int stage, int nvar, void* S,
                  void* tmp, void* t1, void* N, int project, int no_vz_kz0,
                  const double* visc, void* stream) {
+  PlanScope scope_(p);
   if (!p || !S || !t1 || !N || !visc) return fail(FSX_ERR_ARG, "bad argument");
   if (scheme != 4 && scheme != 2) return fail(FSX_ERR_ARG, "scheme must be 2 or 4");
   if (stage < 0 || stage >= scheme) return fail(FSX_ERR_ARG, "bad stage");
@@ -788,6 +894,46 @@ int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
   FSX_HIP(hipMemcpyAsync(host_out, p->d_spec, nbins * sizeof(double),
                          hipMemcpyDeviceToHost, s));
   FSX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsx_timing_enable(fsx_plan p, int enable) {
+  if (!p) return fail(FSX_ERR_ARG, "null plan");
+  if (!p->tm) {
+    p->tm = new fsx_timing();
+    const int npairs = 2048;
+    p->tm->ev.resize(2 * npairs);
+    p->tm->kind.resize(npairs);
+    p->tm->bytes.resize(npairs);
+    for (auto& e : p->tm->ev) FSX_HIP(hipEventCreate(&e));
+  }
+  if (p->tm->on && !enable) {
+    PlanScope scope_(p);
+    int rc = tm_flush(p);
+    if (rc) return rc;
+  }
+  p->tm->on = enable != 0;
+  return 0;
+}
+
+int fsx_timing_read(fsx_plan p, double* ms, double* bytes, long long* counts,
+                    int reset) {
+  if (!p || !ms || !bytes || !counts) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->tm) {
+    for (int k = 0; k < fsx_timing::kKinds; ++k) ms[k] = bytes[k] = 0, counts[k] = 0;
+    return 0;
+  }
+  int rc = tm_flush(p);
+  if (rc) return rc;
+  for (int k = 0; k < fsx_timing::kKinds; ++k) {
+    ms[k] = p->tm->ms[k];
+    bytes[k] = p->tm->by[k];
+    counts[k] = p->tm->cnt[k];
+    if (reset) {
+      p->tm->ms[k] = p->tm->by[k] = 0;
+      p->tm->cnt[k] = 0;
+    }
+  }
   return 0;
 }
 

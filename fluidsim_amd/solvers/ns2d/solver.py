@@ -1,0 +1,52 @@
+"""NS2D solver on the MI355X engine.
+
+Drop-in for fluidsim.solvers.ns2d.solver (fluidsim/solvers/ns2d/solver.py):
+2D incompressible Navier-Stokes in vorticity form,
+d_t rot_hat = N(rot) - sigma(k) rot_hat with N = -u.grad(rot) - beta v.
+"""
+
+from ...base.solvers import SimulBasePseudoSpectral, InfoSolverPseudoSpectral
+from ...setofvariables import SetOfVariables
+
+
+class InfoSolverNS2D(InfoSolverPseudoSpectral):
+    """fluidsim/solvers/ns2d/solver.py:42-92."""
+
+    def _init_root(self):
+        super()._init_root()
+        package = "fluidsim_amd.solvers.ns2d"
+        self.module_name = package + ".solver"
+        self.class_name = "Simul"
+        self.short_name = "NS2D"
+        self._set_class("State", package + ".state", "StateNS2D")
+        self._set_class("InitFields", package + ".init_fields", "InitFieldsNS2D")
+        self._set_class("Output", package + ".output", "OutputNS2D")
+
+
+class Simul(SimulBasePseudoSpectral):
+    InfoSolver = InfoSolverNS2D
+    _engine_projection = 0
+    _cfl_ncomp = 2
+
+    @staticmethod
+    def _complete_params_with_default(params):
+        SimulBasePseudoSpectral._complete_params_with_default(params)
+        params._set_attribs({"beta": 0.0})
+
+    def _engine_tendency(self, rot_fft, out_fft, want_maxima=False):
+        """Fused device tendency of one vorticity field (tensors (1, *K))."""
+        self.oper.ns2d_tendency(rot_fft, out_fft, self.params.beta, want_maxima)
+
+    def tendencies_nonlin(self, state_spect=None, old=None):
+        """fluidsim/solvers/ns2d/solver.py:111-194.
+
+        Returns a SetOfVariables with the dealiased N(rot); ``old`` (if
+        given) is reused as output storage."""
+        if state_spect is None:
+            state_spect = self.state.state_spect
+        tend = SetOfVariables(like=self.state.state_spect) if old is None else old
+        if old is not None:
+            # modes outside the kept box are never written by the engine
+            tend.initialize(0.0)
+        self._engine_tendency(state_spect.data, tend.data)
+        return tend

@@ -8,7 +8,17 @@ fluidsim's ``init_fields.type = "in_script"`` expects
 All random numbers come from ``numpy.random.Generator(PCG64(seed))``.
 """
 
+import os
+
 import numpy as np
+import scipy.fft as sfft
+
+
+def _workers():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return 1
 
 
 def _k_signed(n, dk):
@@ -129,9 +139,11 @@ def ns3d_spectrum_vv_fft(
     dk = 2 * np.pi / L
     _dealias_cubic(vv, [Kx, Ky, Kz], [nx, ny, nz], [dk, dk, dk], coef_dealiasing)
     out = []
+    w = _workers()
     for a in vv:
-        x = np.fft.irfftn(a, s=(nz, ny, nx), axes=(0, 1, 2), norm="forward")
-        out.append(np.fft.rfftn(x, norm="forward"))
+        x = sfft.irfftn(a, s=(nz, ny, nx), axes=(0, 1, 2), norm="forward", workers=w)
+        out.append(sfft.rfftn(x, norm="forward", workers=w))
+    del vv
     E = _sum_wavenumbers(
         0.5 * (np.abs(out[0]) ** 2 + np.abs(out[1]) ** 2 + np.abs(out[2]) ** 2), nx
     )

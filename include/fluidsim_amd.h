@@ -145,6 +145,17 @@ int fsx_sums(fsx_plan p, int kind, const void* f0, const void* f1,
 int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
                  int nf, int nbins, double* host_out, void* stream);
 
+/* Live per-kernel timing (no reference counterpart: measurement support).
+ * When enabled, HIP events are recorded around every pass launch on its
+ * stream.  fsx_timing_read (sync) returns, per launch kind
+ * (0 z-pass inverse, 1 y-pass inverse, 2 x row pass, 3 y-pass forward,
+ * 4 z-pass forward, 5 Runge-Kutta stage, 6 other, 7 unused), the summed
+ * duration in ms, the summed algorithmic bytes (compulsory HBM reads +
+ * writes of each launch) and the launch count; reset zeroes them. */
+int fsx_timing_enable(fsx_plan p, int enable);
+int fsx_timing_read(fsx_plan p, double* ms8, double* bytes8, long long* counts8,
+                    int reset);
+
 #ifdef __cplusplus
 }
 #endif

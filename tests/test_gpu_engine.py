@@ -198,3 +198,46 @@ def test_sums_and_spectrum(prec):
     assert out[1] == pytest.approx(m["Z"], rel=tol)
     assert out[2] == pytest.approx(m["epsK"], rel=tol)
     assert out[4] == pytest.approx(m["epsZ"], rel=tol)
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_projections_vs_oracle(kind):
+    oper = Oper3D(16, 8, 32, 2 * np.pi, 3.0, 5.0)
+    rng = np.random.default_rng(kind)
+    v = np.array([oper.fft(rng.standard_normal(oper.shapeX)) for _ in range(3)])
+    ref = v.copy()
+    {1: oper.project_perpk3d, 2: oper.project_toroidal,
+     3: oper.project_poloidal}[kind](ref[0], ref[1], ref[2])
+    plan = make_plan(oper, 8, 6)
+    d = to_dev_complex(v, 8)
+    _lib.call("fsx_project", plan.handle, _lib.dptr(d[0]), _lib.dptr(d[1]),
+              _lib.dptr(d[2]), kind, S())
+    assert rel_l2(host(d), ref) < 1e-14
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_ns3d_rk4_projection_kinds(kind):
+    proj = {1: None, 2: "toroidal"}[kind]
+    oper = Oper3D(16, 16, 16)
+    sim = NS3D(oper, nu_2=1e-3, projection=proj)
+    vv = ns3d_spectrum_vv_fft(16, seed=3)
+    sim.state_spect[:] = np.array(vv)
+    sim.project_state_spect(sim.state_spect)
+    S0 = sim.state_spect.copy()
+    ref_tend = sim.tendencies_nonlin(S0.copy())
+    plan = make_plan(oper, 8, 6)
+    # tendency + projection
+    v = to_dev_complex(S0, 8)
+    out = zeros_complex((3,) + oper.shapeK, 8)
+    _lib.call("fsx_ns3d_tendency", plan.handle, _lib.dptr(v), _lib.dptr(out), 0, 0.0, 0, S())
+    _lib.call("fsx_project", plan.handle, _lib.dptr(out[0]), _lib.dptr(out[1]),
+              _lib.dptr(out[2]), kind, S())
+    assert rel_l2(host(out), ref_tend) < 1e-12
+    sim.run_with_dts([0.01])
+
+    def tend(src, N):
+        _lib.call("fsx_ns3d_tendency", plan.handle, _lib.dptr(src), _lib.dptr(N),
+                  0, 0.0, 0, S())
+
+    got = _run_engine_rk4(plan, S0, 8, 3, [0.01], [1e-3, 0, 0, 0], tend, project=kind)
+    assert rel_l2(got, sim.state_spect) < 1e-12

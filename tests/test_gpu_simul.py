@@ -1,0 +1,205 @@
+"""GPU tests of the drop-in Simul API against the FP64 oracle.
+
+Bars (written in the tests): FP64 state after N steps within relative L2
+1e-11 of the oracle; FP32 error stated and bounded.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.operators import Oper2D, Oper3D
+from oracle.solvers import NS2D, NS3D
+from fluidsim_amd.synthetic import ns2d_noise_rot_fft, ns3d_spectrum_vv_fft
+
+from .gpu_helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def ns2d_params(n, precision="float64", nu_2=1e-3, it_end=10, cfl=0.5,
+                use_cfl=True):
+    from fluidsim_amd.solvers.ns2d.solver import Simul
+
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = n
+    params.oper.Lx = params.oper.Ly = 2 * np.pi
+    params.oper.precision = precision
+    params.nu_2 = nu_2
+    params.time_stepping.USE_CFL = use_cfl
+    params.time_stepping.cfl_coef = cfl
+    params.time_stepping.USE_T_END = False
+    params.time_stepping.it_end = it_end
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    return Simul, params
+
+
+def test_cfg1_ns2d_256_fp64_200_steps_cfl():
+    """ns2d 256^2 FP64 RK4, nu2 = 1e-3, 2/3 dealiasing, seeded band-limited
+    noise (k <= 16, max|u| = 1), CFL 0.5 adaptive dt, 200 steps:
+    relative L2 of the state <= 1e-11 against the oracle, same dt sequence,
+    and the energy budget closes."""
+    n, nsteps = 256, 200
+    print(f"case cfg1 ns2d n={n} steps={nsteps}", flush=True)
+    rot0 = ns2d_noise_rot_fft(n, n, seed=0, kmax=16)
+    Simul, params = ns2d_params(n, it_end=nsteps)
+    params.output.periods_save.spatial_means = 1e-9  # every step
+    sim = Simul(params)
+    sim.state.init_statespect_from(rot_fft=rot0)
+    dts = []
+    orig = sim.time_stepping.one_time_step_computation
+
+    def record():
+        orig()
+        dts.append(sim.time_stepping.deltat)
+
+    sim.time_stepping.one_time_step_computation = record
+    sim.time_stepping.start()
+    got = sim.state.get_var_numpy("rot_fft")
+
+    orc = NS2D(Oper2D(n, n), nu_2=1e-3, USE_CFL=True, cfl_coef=0.5)
+    orc.init_from_rotfft(rot0)
+    odts = orc.run(nsteps, record_dt=True)
+    err = rel_l2(got, orc.state_spect[0])
+    print(f"cfg1 rel L2 = {err:.3e}; t = {sim.time_stepping.t:.6f}")
+    np.testing.assert_allclose(dts, odts, rtol=1e-13)
+    assert err <= 1e-11
+    # energy budget: dE/dt = -eps (trapezoid between saves), per step
+    m = sim.output.load_spatial_means()
+    E, eps, t = m["E"], m["epsK_tot"], m["t"]
+    dEdt = np.diff(E) / np.diff(t)
+    eps_mid = 0.5 * (eps[1:] + eps[:-1])
+    resid = np.abs(dEdt + eps_mid) / np.abs(eps_mid)
+    print(f"energy budget residual max = {resid.max():.2e}")
+    assert resid.max() < 1e-3
+
+
+def test_ns2d_fp32_error_vs_fp64_oracle():
+    n, nsteps = 128, 50
+    rot0 = ns2d_noise_rot_fft(n, n, seed=1, kmax=16)
+    Simul, params = ns2d_params(n, precision="float32", it_end=nsteps,
+                                use_cfl=False)
+    params.time_stepping.deltat0 = 0.01
+    sim = Simul(params)
+    sim.state.init_statespect_from(rot_fft=rot0)
+    sim.time_stepping.start()
+    got = sim.state.get_var_numpy("rot_fft")
+    orc = NS2D(Oper2D(n, n), nu_2=1e-3, deltat0=0.01)
+    orc.init_from_rotfft(rot0)
+    orc.run(nsteps)
+    err = rel_l2(got, orc.state_spect[0])
+    print(f"ns2d FP32 {n}^2 {nsteps} steps: rel L2 vs FP64 oracle = {err:.3e}")
+    assert err < 1e-4
+
+
+def ns3d_sim(n, precision="float64", nu_2=1e-2, it_end=5, use_cfl=True, cfl=0.5,
+             seed=0, **extra):
+    from fluidsim_amd.solvers.ns3d.solver import Simul
+
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = params.oper.nz = n
+    params.oper.precision = precision
+    params.nu_2 = nu_2
+    for k, v in extra.items():
+        params[k] = v
+    params.time_stepping.USE_CFL = use_cfl
+    params.time_stepping.cfl_coef = cfl
+    params.time_stepping.deltat0 = 0.01
+    params.time_stepping.USE_T_END = False
+    params.time_stepping.it_end = it_end
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    sim = Simul(params)
+    vv = ns3d_spectrum_vv_fft(n, seed=seed)
+    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+    return sim, vv
+
+
+@pytest.mark.parametrize("extra", [{}, {"f": 1.5}, {"projection": "toroidal"}])
+def test_ns3d_fp64_steps_vs_oracle(extra):
+    n, nsteps = 32, 8
+    print(f"case ns3d fp64 n={n} extra={extra}", flush=True)
+    sim, vv = ns3d_sim(n, it_end=nsteps, **extra)
+    orc = NS3D(Oper3D(n, n, n), nu_2=1e-2, USE_CFL=True, cfl_coef=0.5,
+               f=extra.get("f"), projection=extra.get("projection"))
+    orc.state_spect[:] = np.array(vv)
+    if extra.get("projection"):
+        # start both from the projected field
+        orc.project_state_spect(orc.state_spect)
+        sim.project_state_spect(sim.state.state_spect)
+    sim.time_stepping.start()
+    got = sim.state.state_spect.to_numpy()
+    orc.run(nsteps)
+    err = rel_l2(got, orc.state_spect)
+    print(f"ns3d FP64 {n}^3 {nsteps} steps {extra}: rel L2 = {err:.3e}")
+    assert err <= 1e-11
+
+
+def test_ns3d_fp32_error_vs_fp64_oracle():
+    n, nsteps = 64, 10
+    sim, vv = ns3d_sim(n, precision="float32", it_end=nsteps, use_cfl=False,
+                       nu_2=5e-3)
+    sim.time_stepping.start()
+    got = sim.state.state_spect.to_numpy()
+    orc = NS3D(Oper3D(n, n, n), nu_2=5e-3, deltat0=0.01)
+    orc.state_spect[:] = np.array(vv)
+    orc.run(nsteps)
+    err = rel_l2(got, orc.state_spect)
+    print(f"ns3d FP32 {n}^3 {nsteps} steps: rel L2 vs FP64 oracle = {err:.3e}")
+    assert err < 1e-5
+
+
+def test_drop_in_operator_api_2d():
+    Simul, params = ns2d_params(32, use_cfl=False)
+    params.oper.Lx = params.oper.Ly = 6.0
+    params.nu_8 = 2e-5
+    params.init_fields.type = "noise"
+    sim = Simul(params)
+    oper = sim.oper
+    # numpy in, numpy out (fluidfft semantics)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(oper.shapeX)
+    k = oper.fft2(x)
+    assert isinstance(k, np.ndarray)
+    np.testing.assert_allclose(oper.ifft2(k), x, atol=1e-13)
+    ref = Oper2D(32, 32, 6.0, 6.0)
+    np.testing.assert_allclose(k, ref.fft(x), atol=1e-15)
+    # curl round trip (test_operators2d.py:69-80)
+    k[0, 0] = 0
+    ux, uy = oper.vecfft_from_rotfft(k)
+    np.testing.assert_allclose(oper.rotfft_from_vecfft(ux, uy), k, 1e-15, 1e-14)
+    # tendency conservation (ns2d/test_solver.py:56-69) on the noise init
+    rot_fft = sim.state.get_var("rot_fft")
+    tend = sim.tendencies_nonlin(state_spect=sim.state.state_spect)
+    F = tend.get_var("rot_fft")
+    T = (F.conj() * rot_fft).real
+    ratio = oper.sum_wavenumbers(T) / oper.sum_wavenumbers(T.abs())
+    assert abs(ratio) < 1e-14
+    # Parseval consistency of the two representations
+    sim.state.check_energy_equal_phys_spect()
+    sim.time_stepping.start()
+    sim.state.check_energy_equal_phys_spect()
+
+
+def test_nan_raises_value_error():
+    Simul, params = ns2d_params(32, use_cfl=False, it_end=3)
+    params.time_stepping.deltat0 = 0.01
+    sim = Simul(params)
+    rot = ns2d_noise_rot_fft(32, 32, seed=2, kmax=6)
+    rot[1, 1] = np.nan
+    sim.state.init_statespect_from(rot_fft=rot)
+    with pytest.raises(ValueError, match="nan at it"):
+        sim.time_stepping.start()
+
+
+def test_ns3d_tendency_energy_conservation_api():
+    """ns3d/test_solver.py:80-96 through the drop-in API."""
+    sim, _ = ns3d_sim(16, it_end=1)
+    tend = sim.tendencies_nonlin(state_spect=sim.state.state_spect)
+    T = 0
+    for key in ("vx_fft", "vy_fft", "vz_fft"):
+        T = T + (tend.get_var(key).conj() * sim.state.get_var(key)).real
+    ratio = sim.oper.sum_wavenumbers(T) / sim.oper.sum_wavenumbers(T.abs())
+    assert abs(ratio) < 1e-14
