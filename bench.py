@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--precision", default="float32", choices=["float32", "float64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB first "
+                         "(calibration kernel for the HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -134,6 +137,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n, prec = args.n, (4 if args.precision == "float32" else 8)
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(f"[bench] building ns3d {n}^3 {args.precision}", file=sys.stderr, flush=True)
     sim = make_sim(n, args.precision)
     ts = sim.time_stepping

@@ -142,12 +142,21 @@ int tm_flush(fsx_plan_s* p) {
   return 0;
 }
 
-constexpr int kEPT32 = 32;  // 8192 complex64 per tile  = 64 KiB of LDS
-constexpr int kEPT64 = 16;  // 4096 complex128 per tile = 64 KiB of LDS
+constexpr int kThr = 512;   // threads per workgroup of the pass kernels
+constexpr int kEPT32 = 16;  // 8192 complex64 per tile  = 64 KiB of LDS
+constexpr int kEPT64 = 8;   // 4096 complex128 per tile = 64 KiB of LDS
+constexpr int kMinBlocks = 1024;  // shrink tiles of small problems below this
 
 template <typename T>
 constexpr int ept() {
   return sizeof(T) == 4 ? kEPT32 : kEPT64;
+}
+
+// row stride rounded up to 128 bytes
+template <typename T>
+int padded(int m) {
+  const int a = 128 / (int)sizeof(cpx<T>);
+  return (m + a - 1) / a * a;
 }
 
 template <typename T>
@@ -164,16 +173,29 @@ int upload_twiddles(int n, void** out) {
   return 0;
 }
 
+constexpr size_t kLdsMax = 160 * 1024;  // LDS per workgroup on gfx950
+
+template <typename K>
+int set_lds(K kern, size_t lds) {
+  FSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  return 0;
+}
+
 template <typename T, int DIR, class Op>
 int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw,
                    hipStream_t s) {
-  constexpr int E = ept<T>();
-  const int tile = E * kThreads;
+  constexpr int E = ept<T>() / Op::kEptDiv >= 8 ? ept<T>() / Op::kEptDiv : 8;
+  const int tile = E * kThr;
   if (N > tile) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
+  const int nfb = Op::kFieldPerBlock ? nf : 1;
   int L = tile / N;
   int cap = 1;
   while (cap < nlines) cap <<= 1;
   if (L > cap) L = cap;
+  // enough workgroups to fill the chip when the problem is small
+  while (L > 1 && (long long)nfb * nouter * ((nlines + L - 1) / L) < kMinBlocks) L >>= 1;
   op.d.N = N;
   op.d.logN = ilog2(N);
   op.d.L = L;
@@ -186,58 +208,66 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   const long long nchunks = (nlines + L - 1) / L;
   const long long blocks = nchunks * nouter;
   if (blocks > 0x7fffffffLL) return fail(FSX_ERR_ARG, "grid too large");
-  const size_t lds = (size_t)tile * sizeof(cpx<T>);
-  auto kern = k_strided<T, DIR, E, Op>;
-  static bool attr = false;
-  if (!attr) {
-    FSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
-    attr = true;
+  const size_t lds = ((size_t)tile + N) * sizeof(cpx<T>);
+  auto kern = k_strided<T, DIR, E, kThr, Op>;
+  static size_t lds_set = 0;
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
+  if (lds > lds_set) {
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
+    lds_set = lds;
   }
   int rc = tm_pre(s);
   if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, s, op);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks, (unsigned)nfb), dim3(kThr), lds,
+                     s, op);
   FSX_HIP(hipGetLastError());
   return tm_post(s);
 }
 
 template <typename T, class Op>
 int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
-                 int mx_out, T scl, const void* tw, hipStream_t s) {
+                 int ld_in, int mx_out, int ld_out, long long ifs,
+                 long long ofs, T scl, const void* tw, hipStream_t s) {
   constexpr int E = ept<T>();
-  const int tile = E * kThreads;
+  const int tile = E * kThr;
   const int M = nx / 2;
   const int nfm = nf_in > nf_out ? nf_in : nf_out;
   if (nfm * M > tile) return fail(FSX_ERR_ARG, "row pass exceeds the tile");
-  int RB = tile / (nfm * M);
-  if (RB > nrows) RB = nrows;
-  if (RB < 1) RB = 1;
+  int RB = 1;
+  while (2 * RB * nfm * M <= tile && 2 * RB <= nrows) RB *= 2;
+  while (RB > 1 && (nrows + RB - 1) / RB < kMinBlocks) RB >>= 1;
   op.d.nx = nx;
+  op.d.lognx = ilog2(nx);
   op.d.M = M;
   op.d.logM = ilog2(M);
   op.d.RB = RB;
+  op.d.logRB = ilog2(RB);
   op.d.nrows = nrows;
   op.d.nf_in = nf_in;
   op.d.nf_out = nf_out;
   op.d.mx_in = mx_in;
+  op.d.ld_in = ld_in;
   op.d.mx_out = mx_out;
+  op.d.ld_out = ld_out;
+  op.d.ifs = ifs;
+  op.d.ofs = ofs;
   op.d.scale = scl;
   op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
   if (nrows <= 0) return 0;
   const long long blocks = (nrows + RB - 1) / RB;
-  const size_t lds = (size_t)tile * sizeof(cpx<T>);
-  auto kern = k_xpass<T, E, Op>;
-  static bool attr = false;
-  if (!attr) {
-    FSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
-    attr = true;
+  const size_t lds = ((size_t)tile + nx) * sizeof(cpx<T>);
+  auto kern = k_xpass<T, E, kThr, Op>;
+  static size_t lds_set = 0;
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
+  if (lds > lds_set) {
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
+    lds_set = lds;
   }
   int rc = tm_pre(s);
   if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, s, op);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThr), lds, s, op);
   FSX_HIP(hipGetLastError());
   return tm_post(s);
 }
@@ -265,132 +295,149 @@ unsigned grid_for(size_t n) {
   return (unsigned)b;
 }
 
+// Workspace geometry of a pass set (pruned = solver passes, else plain
+// transforms over all modes).
+struct Geo {
+  AxisMap ym, zm;
+  int mx, ld1, ld2;
+  long long fs1, fs2;  // field strides of W1 and W2 (elements)
+};
+
 template <typename T>
-cpx<T>* W1(fsx_plan_s* p, int f, bool pruned) {
-  const size_t fe = pruned ? (size_t)p->nz * p->ymP.nk() * p->mxP
-                           : (size_t)p->nz * p->ny * p->nkx;
-  return reinterpret_cast<cpx<T>*>(p->w1) + f * fe;
-}
-template <typename T>
-cpx<T>* W2(fsx_plan_s* p, int f, bool pruned) {
-  const size_t fe = (size_t)p->nz * p->ny * (pruned ? p->mxP : p->nkx);
-  return reinterpret_cast<cpx<T>*>(p->w2) + f * fe;
+Geo geo(const fsx_plan_s* p, bool pruned) {
+  Geo g;
+  g.ym = pruned ? p->ymP : p->ymF;
+  g.zm = pruned ? p->zmP : p->zmF;
+  g.mx = pruned ? p->mxP : p->mxF;
+  g.ld1 = padded<T>(g.mx);
+  g.ld2 = padded<T>(g.mx);
+  g.fs1 = (long long)p->nz * g.ym.nk() * g.ld1;
+  g.fs2 = (long long)p->nz * p->ny * g.ld2;
+  return g;
 }
 
-// ---------------------------------------------------------------- inverse
-// K (nf fields, stride Kelems) -> W2 (pruned or full) through z and y passes.
 template <typename T>
-int inverse_to_w2(fsx_plan_s* p, const cpx<T>* const* kin, int nf, bool pruned,
-                  hipStream_t s) {
-  const AxisMap ym = pruned ? p->ymP : p->ymF;
-  const AxisMap zm = pruned ? p->zmP : p->zmF;
-  const int mx = pruned ? p->mxP : p->mxF;
-  if (p->dim == 2) {
-    OpYInv2<T> op;
-    for (int f = 0; f < nf; ++f) {
-      op.in[f] = kin[f];
-      op.out[f] = W2<T>(p, f, pruned);
-    }
-    op.nkx = p->nkx;
-    op.mx = mx;
-    op.ym = ym;
-    note(K_YINV, (double)nf * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
-    return launch_strided<T, +1>(op, p->ny, 1, mx, nf, p->tw[1], s);
-  }
-  {
-    OpZInv<T> op;
-    for (int f = 0; f < nf; ++f) {
-      op.in[f] = kin[f];
-      op.out[f] = W1<T>(p, f, pruned);
-    }
-    op.ny = p->ny;
-    op.nkx = p->nkx;
-    op.mx = mx;
-    op.ym = ym;
-    op.zm = zm;
-    note(K_ZINV, (double)nf * ym.nk() * mx * (zm.nk() + p->nz) * sizeof(cpx<T>));
-    int rc = launch_strided<T, +1>(op, p->nz, ym.nk(), mx, nf, p->tw[2], s);
-    if (rc) return rc;
-  }
-  OpYInv3<T> op;
-  for (int f = 0; f < nf; ++f) {
-    op.in[f] = W1<T>(p, f, pruned);
-    op.out[f] = W2<T>(p, f, pruned);
-  }
-  op.ny = p->ny;
-  op.mx = mx;
-  op.ym = ym;
-  note(K_YINV, (double)nf * p->nz * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
-  return launch_strided<T, +1>(op, p->ny, p->nz, mx, nf, p->tw[1], s);
+cpx<T>* W1(const fsx_plan_s* p) {
+  return reinterpret_cast<cpx<T>*>(p->w1);
 }
-
-// W2 (nf fields) -> K through y and z passes; only kept modes are written.
 template <typename T>
-int forward_from_w2(fsx_plan_s* p, cpx<T>* const* kout, int nf, bool pruned,
-                    hipStream_t s) {
-  const AxisMap ym = pruned ? p->ymP : p->ymF;
-  const AxisMap zm = pruned ? p->zmP : p->zmF;
-  const int mx = pruned ? p->mxP : p->mxF;
-  const unsigned char* mask = pruned ? p->d_mask : nullptr;
-  if (p->dim == 2) {
-    OpYFwd2<T> op;
-    for (int f = 0; f < nf; ++f) {
-      op.in[f] = W2<T>(p, f, pruned);
-      op.out[f] = kout[f];
-    }
-    op.nkx = p->nkx;
-    op.mx = mx;
-    op.ym = ym;
-    op.mask = mask;
-    note(K_YFWD, (double)nf * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
-    return launch_strided<T, -1>(op, p->ny, 1, mx, nf, p->tw[1], s);
-  }
-  {
-    OpYFwd3<T> op;
-    for (int f = 0; f < nf; ++f) {
-      op.in[f] = W2<T>(p, f, pruned);
-      op.out[f] = W1<T>(p, f, pruned);
-    }
-    op.ny = p->ny;
-    op.mx = mx;
-    op.ym = ym;
-    note(K_YFWD, (double)nf * p->nz * mx * (ym.nk() + p->ny) * sizeof(cpx<T>));
-    int rc = launch_strided<T, -1>(op, p->ny, p->nz, mx, nf, p->tw[1], s);
-    if (rc) return rc;
-  }
-  OpZFwd<T> op;
-  for (int f = 0; f < nf; ++f) {
-    op.in[f] = W1<T>(p, f, pruned);
-    op.out[f] = kout[f];
-  }
-  op.ny = p->ny;
-  op.nkx = p->nkx;
-  op.mx = mx;
-  op.ym = ym;
-  op.zm = zm;
-  op.mask = mask;
-  note(K_ZFWD, (double)nf * ym.nk() * mx * (p->nz + zm.nk()) * sizeof(cpx<T>));
-  return launch_strided<T, -1>(op, p->nz, ym.nk(), mx, nf, p->tw[2], s);
+cpx<T>* W2(const fsx_plan_s* p) {
+  return reinterpret_cast<cpx<T>*>(p->w2);
 }
 
 size_t kelems(const fsx_plan_s* p) { return (size_t)p->nz * p->ny * p->nkx; }
 size_t xelems(const fsx_plan_s* p) { return (size_t)p->nz * p->ny * p->nx; }
 int nrows(const fsx_plan_s* p) { return p->nz * p->ny; }
 
+// ---------------------------------------------------------------- inverse
+// K (nf fields, field stride kfs) -> W2 through the z and y passes.
+template <typename T>
+int inverse_to_w2(fsx_plan_s* p, const cpx<T>* kin, long long kfs, int nf,
+                  bool pruned, hipStream_t s) {
+  const Geo g = geo<T>(p, pruned);
+  if (p->dim == 2) {
+    OpYInv2<T> op;
+    op.in = kin;
+    op.out = W2<T>(p);
+    op.ifs = kfs;
+    op.ofs = g.fs2;
+    op.nkx = p->nkx;
+    op.ld2 = g.ld2;
+    op.ym = g.ym;
+    note(K_YINV, (double)nf * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+    return launch_strided<T, +1>(op, p->ny, 1, g.mx, nf, p->tw[1], s);
+  }
+  {
+    OpZInv<T> op;
+    op.in = kin;
+    op.out = W1<T>(p);
+    op.ifs = kfs;
+    op.ofs = g.fs1;
+    op.ny = p->ny;
+    op.nkx = p->nkx;
+    op.ld1 = g.ld1;
+    op.ym = g.ym;
+    op.zm = g.zm;
+    note(K_ZINV, (double)nf * g.ym.nk() * g.mx * (g.zm.nk() + p->nz) * sizeof(cpx<T>));
+    int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, nf, p->tw[2], s);
+    if (rc) return rc;
+  }
+  OpYInv3<T> op;
+  op.in = W1<T>(p);
+  op.out = W2<T>(p);
+  op.ifs = g.fs1;
+  op.ofs = g.fs2;
+  op.ny = p->ny;
+  op.ld1 = g.ld1;
+  op.ld2 = g.ld2;
+  op.ym = g.ym;
+  note(K_YINV, (double)nf * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+  return launch_strided<T, +1>(op, p->ny, p->nz, g.mx, nf, p->tw[1], s);
+}
+
+// W2 (nf fields) -> K through the y and z passes; only kept modes written.
+template <typename T>
+int forward_from_w2(fsx_plan_s* p, cpx<T>* kout, long long kfs, int nf,
+                    bool pruned, hipStream_t s) {
+  const Geo g = geo<T>(p, pruned);
+  const unsigned char* mask = pruned ? p->d_mask : nullptr;
+  if (p->dim == 2) {
+    OpYFwd2<T> op;
+    op.in = W2<T>(p);
+    op.out = kout;
+    op.ifs = g.fs2;
+    op.ofs = kfs;
+    op.nkx = p->nkx;
+    op.ld2 = g.ld2;
+    op.ym = g.ym;
+    op.mask = mask;
+    note(K_YFWD, (double)nf * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+    return launch_strided<T, -1>(op, p->ny, 1, g.mx, nf, p->tw[1], s);
+  }
+  {
+    OpYFwd3<T> op;
+    op.in = W2<T>(p);
+    op.out = W1<T>(p);
+    op.ifs = g.fs2;
+    op.ofs = g.fs1;
+    op.ny = p->ny;
+    op.ld1 = g.ld1;
+    op.ld2 = g.ld2;
+    op.ym = g.ym;
+    note(K_YFWD, (double)nf * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+    int rc = launch_strided<T, -1>(op, p->ny, p->nz, g.mx, nf, p->tw[1], s);
+    if (rc) return rc;
+  }
+  OpZFwd<T> op;
+  op.in = W1<T>(p);
+  op.out = kout;
+  op.ifs = g.fs1;
+  op.ofs = kfs;
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.ld1 = g.ld1;
+  op.ym = g.ym;
+  op.zm = g.zm;
+  op.mask = mask;
+  note(K_ZFWD, (double)nf * g.ym.nk() * g.mx * (p->nz + g.zm.nk()) * sizeof(cpx<T>));
+  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, nf, p->tw[2], s);
+}
+
 template <typename T>
 int do_ifft(fsx_plan_s* p, const void* kin, void* xout, int nfields,
             hipStream_t s) {
+  const Geo g = geo<T>(p, false);
   for (int f = 0; f < nfields; ++f) {
     const cpx<T>* k = reinterpret_cast<const cpx<T>*>(kin) + f * kelems(p);
     T* x = reinterpret_cast<T*>(xout) + f * xelems(p);
-    int rc = inverse_to_w2<T>(p, &k, 1, false, s);
+    int rc = inverse_to_w2<T>(p, k, 0, 1, false, s);
     if (rc) return rc;
     OpXC2R<T> op;
-    op.in[0] = W2<T>(p, 0, false);
-    op.outr[0] = x;
+    op.in = W2<T>(p);
+    op.outr = x;
     note(K_XPASS, (double)nrows(p) * (p->mxF * sizeof(cpx<T>) + p->nx * sizeof(T)));
-    rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, p->mxF, T(1),
-                         p->tw[0], s);
+    rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, g.ld2, p->mxF,
+                         g.ld2, 0, 0, T(1), p->tw[0], s);
     if (rc) return rc;
   }
   return 0;
@@ -399,18 +446,19 @@ int do_ifft(fsx_plan_s* p, const void* kin, void* xout, int nfields,
 template <typename T>
 int do_fft(fsx_plan_s* p, const void* xin, void* kout, int nfields,
            hipStream_t s) {
+  const Geo g = geo<T>(p, false);
   const double npts = (double)p->nx * p->ny * p->nz;
   for (int f = 0; f < nfields; ++f) {
     const T* x = reinterpret_cast<const T*>(xin) + f * xelems(p);
     cpx<T>* k = reinterpret_cast<cpx<T>*>(kout) + f * kelems(p);
     OpXR2C<T> op;
-    op.in[0] = x;
-    op.out[0] = W2<T>(p, 0, false);
+    op.inr = x;
+    op.out = W2<T>(p);
     note(K_XPASS, (double)nrows(p) * (p->mxF * sizeof(cpx<T>) + p->nx * sizeof(T)));
-    int rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, p->mxF,
-                             T(1.0 / npts), p->tw[0], s);
+    int rc = launch_xpass<T>(op, p->nx, nrows(p), 1, 1, p->mxF, g.ld2, p->mxF,
+                             g.ld2, 0, 0, T(1.0 / npts), p->tw[0], s);
     if (rc) return rc;
-    rc = forward_from_w2<T>(p, &k, 1, false, s);
+    rc = forward_from_w2<T>(p, k, 0, 1, false, s);
     if (rc) return rc;
   }
   return 0;
@@ -419,80 +467,82 @@ int do_fft(fsx_plan_s* p, const void* xin, void* kout, int nfields,
 template <typename T>
 int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
             int want_maxima, hipStream_t s) {
+  const Geo g = geo<T>(p, true);
   OpYInv2NS<T> op;
-  op.in[0] = reinterpret_cast<const cpx<T>*>(rot);
-  for (int f = 0; f < 4; ++f) op.out[f] = W2<T>(p, f, true);
+  op.in = reinterpret_cast<const cpx<T>*>(rot);
+  op.out = W2<T>(p);
+  op.ofs = g.fs2;
   op.ny = p->ny;
   op.nkx = p->nkx;
-  op.mx = p->mxP;
-  op.ym = p->ymP;
+  op.ld2 = g.ld2;
+  op.ym = g.ym;
   op.dkx = (T)p->dkx;
   op.dky = (T)p->dky;
-  note(K_YINV, (double)p->mxP * (p->ymP.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
-  int rc = launch_strided<T, +1>(op, p->ny, 1, p->mxP, 4, p->tw[1], s);
+  note(K_YINV, (double)g.mx * (g.ym.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
+  int rc = launch_strided<T, +1>(op, p->ny, 1, g.mx, 4, p->tw[1], s);
   if (rc) return rc;
-  OpXNS2D<T> ox;
-  for (int f = 0; f < 4; ++f) ox.in[f] = W2<T>(p, f, true);
-  ox.out[0] = W2<T>(p, 0, true);
+  OpXNS2D<T, kThr> ox;
+  ox.in = W2<T>(p);
+  ox.out = W2<T>(p);
   ox.maxima =
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
   ox.beta = (T)beta;
   const double npts = (double)p->nx * p->ny;
-  note(K_XPASS, 5.0 * p->ny * p->mxP * sizeof(cpx<T>));
-  rc = launch_xpass<T>(ox, p->nx, p->ny, 4, 1, p->mxP, p->mxP, T(1.0 / npts),
-                       p->tw[0], s);
+  note(K_XPASS, 5.0 * p->ny * g.mx * sizeof(cpx<T>));
+  rc = launch_xpass<T>(ox, p->nx, p->ny, 4, 1, g.mx, g.ld2, g.mx, g.ld2, g.fs2,
+                       g.fs2, T(1.0 / npts), p->tw[0], s);
   if (rc) return rc;
-  cpx<T>* k = reinterpret_cast<cpx<T>*>(out);
-  return forward_from_w2<T>(p, &k, 1, true, s);
+  return forward_from_w2<T>(p, reinterpret_cast<cpx<T>*>(out), 0, 1, true, s);
 }
 
 template <typename T>
 int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
             int want_maxima, hipStream_t s) {
-  const size_t ke = kelems(p);
+  const Geo g = geo<T>(p, true);
+  const long long ke = (long long)kelems(p);
   OpZInvCurl<T> op;
-  for (int f = 0; f < 3; ++f)
-    op.in[f] = reinterpret_cast<const cpx<T>*>(v) + f * ke;
-  for (int f = 0; f < 6; ++f) op.out[f] = W1<T>(p, f, true);
+  op.in = reinterpret_cast<const cpx<T>*>(v);
+  op.out = W1<T>(p);
+  op.ifs = ke;
+  op.ofs = g.fs1;
   op.ny = p->ny;
   op.nz = p->nz;
   op.nkx = p->nkx;
-  op.mx = p->mxP;
-  op.ym = p->ymP;
-  op.zm = p->zmP;
+  op.ld1 = g.ld1;
+  op.ym = g.ym;
+  op.zm = g.zm;
   op.dkx = (T)p->dkx;
   op.dky = (T)p->dky;
   op.dkz = (T)p->dkz;
   op.has_f = has_f;
   op.fcor = (T)fcor;
-  note(K_ZINV, (double)p->ymP.nk() * p->mxP * (3.0 * p->zmP.nk() + 6.0 * p->nz) *
+  note(K_ZINV, (double)g.ym.nk() * g.mx * (3.0 * g.zm.nk() + 6.0 * p->nz) *
                    sizeof(cpx<T>));
-  int rc = launch_strided<T, +1>(op, p->nz, p->ymP.nk(), p->mxP, 6, p->tw[2], s);
+  int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, 6, p->tw[2], s);
   if (rc) return rc;
   OpYInv3<T> oy;
-  for (int f = 0; f < 6; ++f) {
-    oy.in[f] = W1<T>(p, f, true);
-    oy.out[f] = W2<T>(p, f, true);
-  }
+  oy.in = W1<T>(p);
+  oy.out = W2<T>(p);
+  oy.ifs = g.fs1;
+  oy.ofs = g.fs2;
   oy.ny = p->ny;
-  oy.mx = p->mxP;
-  oy.ym = p->ymP;
-  note(K_YINV, 6.0 * p->nz * p->mxP * (p->ymP.nk() + p->ny) * sizeof(cpx<T>));
-  rc = launch_strided<T, +1>(oy, p->ny, p->nz, p->mxP, 6, p->tw[1], s);
+  oy.ld1 = g.ld1;
+  oy.ld2 = g.ld2;
+  oy.ym = g.ym;
+  note(K_YINV, 6.0 * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+  rc = launch_strided<T, +1>(oy, p->ny, p->nz, g.mx, 6, p->tw[1], s);
   if (rc) return rc;
-  OpXNS3D<T> ox;
-  for (int f = 0; f < 6; ++f) ox.in[f] = W2<T>(p, f, true);
-  for (int f = 0; f < 3; ++f) ox.out[f] = W2<T>(p, f, true);
+  OpXNS3D<T, kThr> ox;
+  ox.in = W2<T>(p);
+  ox.out = W2<T>(p);
   ox.maxima =
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
   const double npts = (double)p->nx * p->ny * p->nz;
-  note(K_XPASS, 9.0 * nrows(p) * p->mxP * sizeof(cpx<T>));
-  rc = launch_xpass<T>(ox, p->nx, nrows(p), 6, 3, p->mxP, p->mxP,
-                       T(1.0 / npts), p->tw[0], s);
+  note(K_XPASS, 9.0 * nrows(p) * g.mx * sizeof(cpx<T>));
+  rc = launch_xpass<T>(ox, p->nx, nrows(p), 6, 3, g.mx, g.ld2, g.mx, g.ld2,
+                       g.fs2, g.fs2, T(1.0 / npts), p->tw[0], s);
   if (rc) return rc;
-  cpx<T>* k[3];
-  for (int f = 0; f < 3; ++f) k[f] = reinterpret_cast<cpx<T>*>(out) + f * ke;
-  return forward_from_w2<T>(p, k, 3, true, s);
+  return forward_from_w2<T>(p, reinterpret_cast<cpx<T>*>(out), ke, 3, true, s);
 }
 
 template <typename T>
@@ -576,7 +626,7 @@ int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
       (dim == 3 && nz < 2))
     return fail(FSX_ERR_ARG,
                 "the engine supports power-of-two grids (nx >= 4, ny, nz >= 2)");
-  const int tile = (precision == 4 ? kEPT32 : kEPT64) * kThreads;
+  const int tile = (precision == 4 ? kEPT32 : kEPT64) * kThr;
   if (ny > tile || nz > tile || nx / 2 > tile)
     return fail(FSX_ERR_ARG, "grid dimension too large for the LDS tile");
   if (nf_max < 1 || nf_max > 8) return fail(FSX_ERR_ARG, "nf_max must be 1..8");
@@ -658,9 +708,12 @@ int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
     fsx_plan_destroy(p);
     return rc;
   }
-  const size_t full = (size_t)nz * ny * p->nkx;
-  const size_t w1p = (size_t)nf_max * nz * p->ymP.nk() * p->mxP;
-  const size_t w2p = (size_t)nf_max * nz * ny * p->mxP;
+  const int align = 128 / (2 * precision);
+  const size_t ldF = (size_t)((p->mxF + align - 1) / align * align);
+  const size_t ldP = (size_t)((p->mxP + align - 1) / align * align);
+  const size_t full = (size_t)nz * ny * ldF;
+  const size_t w1p = (size_t)nf_max * nz * p->ymP.nk() * ldP;
+  const size_t w2p = (size_t)nf_max * nz * ny * ldP;
   p->w1_elems = dim == 3 ? (w1p > full ? w1p : full) : 0;
   p->w2_elems = w2p > full ? w2p : full;
   const size_t csz = 2 * (size_t)precision;

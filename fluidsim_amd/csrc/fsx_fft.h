@@ -4,20 +4,25 @@
 // layout numpy's complex64/complex128 use, so K arrays cross the C-ABI as
 // plain pointers.
 //
-// The transform is a Stockham autosort FFT of power-of-two length N done in
-// LDS: a tile of `lines` lines is staged in LDS, every radix-R stage reads its
-// R inputs per butterfly into registers, applies twiddles and a hard-coded
-// DFT_R, and writes the outputs back in Stockham order, so the result is in
-// natural order without a bit-reversal pass.  Stage radices are 8 while at
-// least three factors of two remain, then 4 or 2.
+// Transforms are Stockham autosort FFTs of power-of-two length N: radix-R
+// stages (R = 8 while at least three factors of two remain, then 4 or 2).
+// Butterfly j of a stage with sub-transform length Ns reads positions
+// j + r N/R, multiplies input r by W_{Ns R}^{r k} (k = j mod Ns), applies a
+// hard-coded DFT_R and writes output r to (j - k) R + k + r Ns, so the
+// result is in natural order without a bit-reversal pass.
+//
+// The passes keep the first stage in registers (inputs come straight from
+// HBM) and the last stage in registers (outputs go straight to HBM: for the
+// last stage k = j, so output r lands at position j + r Ns); only the middle
+// stages exchange data through LDS.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace fsx {
+#include <type_traits>
 
-constexpr int kThreads = 256;  // threads per workgroup for every pass kernel
+namespace fsx {
 
 template <typename T>
 struct cpx {
@@ -97,7 +102,6 @@ __device__ __forceinline__ void dft8(cpx<T>* v) {
     b[k] = v[k] + v[k + 4];
     c[k] = v[k] - v[k + 4];
   }
-  // c[k] *= w8^k with w8 = exp(DIR i pi/4)
   {
     cpx<T> a = c[1];
     c[1] = DIR < 0 ? mk<T>((a.x + a.y) * h, (a.y - a.x) * h)
@@ -126,94 +130,121 @@ __device__ __forceinline__ void dftR(cpx<T>* v) {
     dft2<T, DIR>(v);
 }
 
-// One Stockham stage of radix R on `lines` lines of length N = 2^logN stored
-// in LDS at buf[pos * ps + line * ls].  Ns = 2^logNs is the length of the
-// sub-transforms already done.  Butterfly j of a line reads positions
-// j + r N/R, multiplies input r by W_{Ns R}^{r k} (k = j mod Ns) and writes
-// output r to ((j - k) R + k) + r Ns.
-//
-// Twiddles come from a global table tw[m] = exp(-2 pi i m / (N << twshift)).
-// Each thread owns EPT/R butterflies; the whole tile must hold at most
-// EPT * kThreads elements.  line_fast selects the thread -> butterfly map:
-// consecutive threads take consecutive lines (strided passes, lines are
-// contiguous in LDS) or consecutive butterflies (row passes).
-template <typename T, int DIR, int EPT, int R>
-__device__ __forceinline__ void fft_stage(cpx<T>* buf, int logN, int logNs,
-                                          int lines, bool line_fast, int ps,
-                                          int ls, const cpx<T>* __restrict__ tw,
-                                          int twshift) {
+// Radix of stage s of a length-2^logN transform (8s first, then 4 or 2).
+__host__ __device__ __forceinline__ int stage_log_radix(int logN, int logNs) {
+  const int rem = logN - logNs;
+  return rem >= 3 ? 3 : rem;
+}
+
+// Apply the Stockham twiddles W_{Ns R}^{r k} to v[1..R-1].
+// tw is a table exp(-2 pi i m / Ntab) with Ntab = N << twshift.
+template <typename T, int DIR, int R, int LOGR>
+__device__ __forceinline__ void twiddle(cpx<T>* v, int k, int logN, int logNs,
+                                        const cpx<T>* tw, int twshift) {
+  const int sh = (logN - logNs - LOGR) + twshift;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    cpx<T> w = tw[(r * k) << sh];
+    if (DIR > 0) w.y = -w.y;
+    v[r] = v[r] * w;
+  }
+}
+
+// One middle Stockham stage in LDS over `lines` lines stored at
+// buf[(pos << logL) + line] (line-fast: lines are contiguous) with
+// 2^logL lines.  Each of the kT threads owns EPT/R butterflies.
+template <typename T, int DIR, int EPT, int R, int kT>
+__device__ __forceinline__ void lds_stage_lf(cpx<T>* buf, int logN, int logNs,
+                                             int logL, const cpx<T>* tw) {
   constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
   constexpr int NB = EPT / R;
   const int lognb = logN - LOGR;
-  const int nb = 1 << lognb;
-  const int total = nb * lines;
+  const int total = 1 << (lognb + logL);
   const int Ns = 1 << logNs;
-  const int tid = threadIdx.x;
   cpx<T> v[NB][R];
-  int obase[NB];
-  bool act[NB];
+  int ob[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    const int b = tid + i * kThreads;
-    act[i] = b < total;
-    if (act[i]) {
-      int j, line;
-      if (line_fast) {
-        line = b % lines;
-        j = b / lines;
-      } else {
-        j = b & (nb - 1);
-        line = b >> lognb;
-      }
+    const int b = threadIdx.x + i * kT;
+    if (b < total) {
+      const int line = b & ((1 << logL) - 1);
+      const int j = b >> logL;
       const int k = j & (Ns - 1);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * nb) * ps + line * ls];
-      if (logNs > 0) {
-        const int sh = (logN - logNs - LOGR) + twshift;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          cpx<T> w = tw[(r * k) << sh];
-          if (DIR > 0) w.y = -w.y;
-          v[i][r] = v[i][r] * w;
-        }
-      }
+      for (int r = 0; r < R; ++r)
+        v[i][r] = buf[((j + (r << lognb)) << logL) + line];
+      twiddle<T, DIR, R, LOGR>(v[i], k, logN, logNs, tw, 0);
       dftR<T, DIR, R>(v[i]);
-      obase[i] = (((j - k) << LOGR) + k) * ps + line * ls;
+      ob[i] = ((((j - k) << LOGR) + k) << logL) + line;
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    if (act[i]) {
+    const int b = threadIdx.x + i * kT;
+    if (b < total) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[obase[i] + r * Ns * ps] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[ob[i] + ((r * Ns) << logL)] = v[i][r];
     }
   }
   __syncthreads();
 }
 
-// Full in-LDS FFT (all stages).  Caller synchronises before the call.
-template <typename T, int DIR, int EPT>
-__device__ __forceinline__ void lds_fft(cpx<T>* buf, int logN, int lines,
-                                        bool line_fast, int ps, int ls,
-                                        const cpx<T>* __restrict__ tw,
-                                        int twshift) {
+// Full in-LDS FFT over `lines` rows stored at buf[line * ldr + pos]
+// (row layout, butterfly-fast thread map), all stages.  Used by the row
+// (x) pass.  Caller synchronises before the call.
+template <typename T, int DIR, int EPT, int R, int kT>
+__device__ __forceinline__ void lds_stage_rows(cpx<T>* buf, int logN,
+                                               int logNs, int lines, int ldr,
+                                               const cpx<T>* tw, int twshift) {
+  constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  constexpr int NB = EPT / R;
+  const int lognb = logN - LOGR;
+  const int total = lines << lognb;
+  const int Ns = 1 << logNs;
+  cpx<T> v[NB][R];
+  int ob[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = threadIdx.x + i * kT;
+    if (b < total) {
+      const int j = b & ((1 << lognb) - 1);
+      const int line = b >> lognb;
+      const int k = j & (Ns - 1);
+      const cpx<T>* row = buf + line * ldr;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = row[j + (r << lognb)];
+      if (logNs > 0) twiddle<T, DIR, R, LOGR>(v[i], k, logN, logNs, tw, twshift);
+      dftR<T, DIR, R>(v[i]);
+      ob[i] = line * ldr + ((j - k) << LOGR) + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = threadIdx.x + i * kT;
+    if (b < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[ob[i] + r * Ns] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int DIR, int EPT, int kT>
+__device__ __forceinline__ void lds_fft_rows(cpx<T>* buf, int logN, int lines,
+                                             int ldr, const cpx<T>* tw,
+                                             int twshift) {
   int logNs = 0;
   while (logNs < logN) {
-    const int rem = logN - logNs;
-    if (rem >= 3) {
-      fft_stage<T, DIR, EPT, 8>(buf, logN, logNs, lines, line_fast, ps, ls, tw,
-                                twshift);
-      logNs += 3;
-    } else if (rem == 2) {
-      fft_stage<T, DIR, EPT, 4>(buf, logN, logNs, lines, line_fast, ps, ls, tw,
-                                twshift);
-      logNs += 2;
-    } else {
-      fft_stage<T, DIR, EPT, 2>(buf, logN, logNs, lines, line_fast, ps, ls, tw,
-                                twshift);
-      logNs += 1;
-    }
+    const int lr = stage_log_radix(logN, logNs);
+    if (lr == 3)
+      lds_stage_rows<T, DIR, EPT, 8, kT>(buf, logN, logNs, lines, ldr, tw, twshift);
+    else if (lr == 2)
+      lds_stage_rows<T, DIR, EPT, 4, kT>(buf, logN, logNs, lines, ldr, tw, twshift);
+    else
+      lds_stage_rows<T, DIR, EPT, 2, kT>(buf, logN, logNs, lines, ldr, tw, twshift);
+    logNs += lr;
   }
 }
 

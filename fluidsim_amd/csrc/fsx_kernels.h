@@ -3,18 +3,21 @@
 // Array layouts in HBM (C order, complex = interleaved re/im):
 //   X  (physical)     : (nz, ny, nx) real            (2D: nz = 1)
 //   K  (spectral)     : (nz, ny, nkx) complex, nkx = nx/2 + 1
-//   W1 (after z-pass) : (nz, nyk, mx) complex        (3D only)
-//   W2 (after y-pass) : (nz, ny,  mx) complex
-// where mx is the number of kept kx indices and nyk the number of kept ky
-// indices of the pass (everything for plain transforms, the bounding box of
-// the non-dealiased modes for the solver passes, whose pruned modes are zero
-// and are neither loaded nor stored).
+//   W1 (after z-pass) : (nz, nyk, ld1) complex       (3D only)
+//   W2 (after y-pass) : (nz, ny,  ld2) complex
+// mx is the number of kept kx indices and nyk the number of kept ky indices
+// of the pass (everything for plain transforms, the bounding box of the
+// non-dealiased modes for the solver passes, whose pruned modes are zero and
+// are neither loaded nor stored).  ld1, ld2 >= mx are row strides rounded up
+// to 128 bytes so that every run of a tile starts on a cache line.
 //
-// A strided pass transforms along y or z: a workgroup stages a tile of L
-// adjacent lines (consecutive kx, the contiguous index) of length N in LDS,
-// so every global access is a contiguous run of L complex numbers.  A row
-// pass transforms along x (real <-> half complex through a complex FFT of
-// length nx/2) and is where the physical-space products are fused.
+// A strided pass transforms along y or z: a workgroup owns L adjacent lines
+// (consecutive kx, the contiguous index) of length N, so every global access
+// is a contiguous run of L complex numbers.  The first Stockham stage reads
+// its butterfly inputs straight from HBM into registers, the middle stages
+// go through LDS, the last stage writes straight to HBM.  A row pass
+// transforms along x (real <-> half complex through a complex FFT of length
+// nx/2) and is where the physical-space products are fused.
 #pragma once
 
 #include "fsx_fft.h"
@@ -28,236 +31,417 @@ struct StridedDesc {
   int nouter;     // number of independent planes
   int nlines;     // lines per plane
   int nf;         // fields transformed one after the other
-  const cpx<T>* tw;  // exp(-2 pi i m / N), m < N
+  const cpx<T>* tw;  // exp(-2 pi i m / N), m < N (copied to LDS)
 };
 
-// ---------------------------------------------------------------- strided
-template <typename T, int DIR, int EPT, class Op>
-__global__ __launch_bounds__(kThreads) void k_strided(Op op) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
-  const StridedDesc<T>& d = op.d;
-  const int nchunks = (d.nlines + d.L - 1) >> d.logL;
-  const int o = blockIdx.x / nchunks;
-  const int c = blockIdx.x - o * nchunks;
-  const int tid = threadIdx.x;
-  const int ntile = d.N << d.logL;
-  for (int f = 0; f < d.nf; ++f) {
+// Full transform of one tile.  src(e, l, p) gives the input at position p
+// of local line l (e = register slot of the first stage, stable across
+// calls); dst(e, l, p, v) receives the output at position p (e = register
+// slot of the last stage).  Ends with a barrier so buf can be reused.
+template <typename T, int DIR, int EPT, int kT, class Src, class Dst>
+__device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
+                                         int logN, int logL, Src&& src,
+                                         Dst&& dst) {
+  const int lmask = (1 << logL) - 1;
+  if (logN <= 3) {
+    // single stage: HBM -> registers -> HBM
+    auto single = [&](auto Rc) {
+      constexpr int R = decltype(Rc)::value;
+      constexpr int NB = EPT / R;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int idx = tid + i * kThreads;
-      if (idx < ntile) {
-        const int p = idx >> d.logL;
-        const int line = (c << d.logL) + (idx & (d.L - 1));
-        buf[idx] = line < d.nlines ? op.load(f, o, line, p) : zero<T>();
+      for (int i = 0; i < NB; ++i) {
+        const int b = threadIdx.x + i * kT;
+        if (b <= lmask) {
+          cpx<T> v[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[r] = src(i * R + r, b, r);
+          dftR<T, DIR, R>(v);
+#pragma unroll
+          for (int r = 0; r < R; ++r) dst(i * R + r, b, r, v[r]);
+        }
       }
-    }
+    };
+    if (logN == 3)
+      single(std::integral_constant<int, 8>());
+    else if (logN == 2)
+      single(std::integral_constant<int, 4>());
+    else
+      single(std::integral_constant<int, 2>());
     __syncthreads();
-    lds_fft<T, DIR, EPT>(buf, d.logN, d.L, true, d.L, 1, d.tw, 0);
+    return;
+  }
+  // first stage, radix 8, Ns = 1: inputs from HBM, outputs to LDS
+  {
+    constexpr int NB = EPT / 8;
+    const int lognb = logN - 3;
+    const int total = 1 << (lognb + logL);
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int idx = tid + i * kThreads;
-      if (idx < ntile) {
-        const int p = idx >> d.logL;
-        const int line = (c << d.logL) + (idx & (d.L - 1));
-        if (line < d.nlines) op.store(f, o, line, p, buf[idx]);
+    for (int i = 0; i < NB; ++i) {
+      const int b = threadIdx.x + i * kT;
+      if (b < total) {
+        const int line = b & lmask;
+        const int j = b >> logL;
+        cpx<T> v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = src(i * 8 + r, line, j + (r << lognb));
+        dft8<T, DIR>(v);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[(((j << 3) + r) << logL) + line] = v[r];
       }
     }
     __syncthreads();
   }
+  int logNs = 3;
+  while (logN - logNs > 3) {
+    lds_stage_lf<T, DIR, EPT, 8, kT>(buf, logN, logNs, logL, tw);
+    logNs += 3;
+  }
+  // last stage: k = j, outputs at j + r Ns, to HBM
+  auto last = [&](auto Rc) {
+    constexpr int R = decltype(Rc)::value;
+    constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+    constexpr int NB = EPT / R;
+    const int lognb = logN - LOGR;
+    const int total = 1 << (lognb + logL);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = threadIdx.x + i * kT;
+      if (b < total) {
+        const int line = b & lmask;
+        const int j = b >> logL;
+        cpx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = buf[((j + (r << lognb)) << logL) + line];
+        twiddle<T, DIR, R, LOGR>(v, j, logN, logN - LOGR, tw, 0);
+        dftR<T, DIR, R>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst(i * R + r, line, j + (r << lognb), v[r]);
+      }
+    }
+  };
+  const int rem = logN - logNs;
+  if (rem == 3)
+    last(std::integral_constant<int, 8>());
+  else if (rem == 2)
+    last(std::integral_constant<int, 4>());
+  else
+    last(std::integral_constant<int, 2>());
+  __syncthreads();
 }
 
-// Plain z-pass, inverse: K (nz, ny, nkx) -> W1 (nz, nyk, mx).
+// ---------------------------------------------------------------- strided
+template <typename T, int DIR, int EPT, int kT, class Op>
+__global__ __launch_bounds__(kT) void k_strided(Op op) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const StridedDesc<T>& d = op.d;
+  cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* buf = s_tw + d.N;
+  for (int i = threadIdx.x; i < d.N; i += kT) s_tw[i] = d.tw[i];
+  // (tile_fft's first barrier precedes every twiddle read)
+  const int nchunks = (d.nlines + d.L - 1) >> d.logL;
+  const int o = blockIdx.x / nchunks;
+  const int line0 = (blockIdx.x - o * nchunks) << d.logL;
+  op.template run<DIR, EPT, kT>(buf, s_tw, o, line0);
+}
+
+// Plain pass: one field per workgroup (blockIdx.y), load()/store().
+template <typename T, class Self>
+struct PlainRun {
+  static constexpr int kEptDiv = 1;
+  static constexpr bool kFieldPerBlock = true;
+  template <int DIR, int EPT, int kT>
+  __device__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+    const Self& s = static_cast<const Self&>(*this);
+    const int nlines = s.d.nlines;
+    const int f = blockIdx.y;
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, s.d.logN, s.d.logL,
+        [&](int, int l, int p) {
+          const int line = line0 + l;
+          return line < nlines ? s.load(f, o, line, p) : zero<T>();
+        },
+        [&](int, int l, int p, cpx<T> v) {
+          const int line = line0 + l;
+          if (line < nlines) s.store(f, o, line, p, v);
+        });
+  }
+};
+
+// Plain z-pass, inverse: K (nz, ny, nkx) -> W1 (nz, nyk, ld1).
 // Lines are kx in [0, mx); planes are the kept ky rows.
 template <typename T>
-struct OpZInv {
+struct OpZInv : PlainRun<T, OpZInv<T>> {
   StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int ny, nkx, mx;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ifs, ofs;  // field strides (elements)
+  int ny, nkx, ld1;
   AxisMap ym, zm;
   __device__ cpx<T> load(int f, int o, int l, int p) const {
     if (zm.compact(p) < 0) return zero<T>();
-    const int iy = ym.full(o);
-    return in[f][((size_t)p * ny + iy) * nkx + l];
+    return in[f * ifs + (p * ny + ym.full(o)) * nkx + l];
   }
   __device__ void store(int f, int o, int l, int p, cpx<T> v) const {
-    out[f][((size_t)p * ym.nk() + o) * mx + l] = v;
+    out[f * ofs + (p * ym.nk() + o) * ld1 + l] = v;
   }
 };
 
-// ns3d z-pass, inverse, with the curl folded into the load:
-// fields 0-2 are v, fields 3-5 are omega = i k x v (+ f at k = 0).
+// y-pass, inverse: W1 (nz, nyk, ld1) -> W2 (nz, ny, ld2); planes are z.
 template <typename T>
-struct OpZInvCurl {
+struct OpYInv3 : PlainRun<T, OpYInv3<T>> {
   StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int ny, nz, nkx, mx;
-  AxisMap ym, zm;
-  T dkx, dky, dkz;
-  int has_f;
-  T fcor;
-  __device__ cpx<T> load(int f, int o, int l, int p) const {
-    if (zm.compact(p) < 0) return zero<T>();
-    const int iy = ym.full(o);
-    const size_t idx = ((size_t)p * ny + iy) * nkx + l;
-    if (f < 3) return in[f][idx];
-    const T kx = T(l) * dkx;
-    const T ky = T(ksigned(iy, ny)) * dky;
-    const T kz = T(ksigned(p, nz)) * dkz;
-    cpx<T> r;
-    if (f == 3) {
-      const cpx<T> vy = in[1][idx], vz = in[2][idx];
-      r = mul_i(scale(vz, ky) - scale(vy, kz));
-    } else if (f == 4) {
-      const cpx<T> vx = in[0][idx], vz = in[2][idx];
-      r = mul_i(scale(vx, kz) - scale(vz, kx));
-    } else {
-      const cpx<T> vx = in[0][idx], vy = in[1][idx];
-      r = mul_i(scale(vy, kx) - scale(vx, ky));
-      if (has_f && idx == 0) r.x += fcor;
-    }
-    return r;
-  }
-  __device__ void store(int f, int o, int l, int p, cpx<T> v) const {
-    out[f][((size_t)p * ym.nk() + o) * mx + l] = v;
-  }
-};
-
-// y-pass, inverse: W1 (nz, nyk, mx) -> W2 (nz, ny, mx); planes are z.
-template <typename T>
-struct OpYInv3 {
-  StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int ny, mx;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ifs, ofs;
+  int ny, ld1, ld2;
   AxisMap ym;
   __device__ cpx<T> load(int f, int z, int l, int p) const {
     const int jy = ym.compact(p);
     if (jy < 0) return zero<T>();
-    return in[f][((size_t)z * ym.nk() + jy) * mx + l];
+    return in[f * ifs + (z * ym.nk() + jy) * ld1 + l];
   }
   __device__ void store(int f, int z, int l, int p, cpx<T> v) const {
-    out[f][((size_t)z * ny + p) * mx + l] = v;
+    out[f * ofs + (z * ny + p) * ld2 + l] = v;
   }
 };
 
-// y-pass, forward: W2 (nz, ny, mx) -> W1 (nz, nyk, mx).
+// y-pass, forward: W2 (nz, ny, ld2) -> W1 (nz, nyk, ld1).
 template <typename T>
-struct OpYFwd3 {
+struct OpYFwd3 : PlainRun<T, OpYFwd3<T>> {
   StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int ny, mx;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ifs, ofs;
+  int ny, ld1, ld2;
   AxisMap ym;
   __device__ cpx<T> load(int f, int z, int l, int p) const {
-    return in[f][((size_t)z * ny + p) * mx + l];
+    return in[f * ifs + (z * ny + p) * ld2 + l];
   }
   __device__ void store(int f, int z, int l, int p, cpx<T> v) const {
     const int jy = ym.compact(p);
-    if (jy >= 0) out[f][((size_t)z * ym.nk() + jy) * mx + l] = v;
+    if (jy >= 0) out[f * ofs + (z * ym.nk() + jy) * ld1 + l] = v;
   }
 };
 
-// z-pass, forward: W1 (nz, nyk, mx) -> K (nz, ny, nkx); only kept modes are
-// written, dealiased ones (mask) as zero.
+// z-pass, forward: W1 (nz, nyk, ld1) -> K (nz, ny, nkx); only kept modes
+// are written, dealiased ones (mask) as zero.
 template <typename T>
-struct OpZFwd {
+struct OpZFwd : PlainRun<T, OpZFwd<T>> {
   StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int ny, nkx, mx;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ifs, ofs;
+  int ny, nkx, ld1;
   AxisMap ym, zm;
   const unsigned char* mask;
   __device__ cpx<T> load(int f, int jy, int l, int p) const {
-    return in[f][((size_t)p * ym.nk() + jy) * mx + l];
+    return in[f * ifs + (p * ym.nk() + jy) * ld1 + l];
   }
   __device__ void store(int f, int jy, int l, int p, cpx<T> v) const {
     if (zm.compact(p) < 0) return;
-    const int iy = ym.full(jy);
-    const size_t idx = ((size_t)p * ny + iy) * nkx + l;
+    const int idx = (p * ny + ym.full(jy)) * nkx + l;
     if (mask && mask[idx]) v = zero<T>();
-    out[f][idx] = v;
+    out[f * ofs + idx] = v;
   }
 };
 
-// 2D y-pass, inverse: K (ny, nkx) -> W2 (ny, mx).
+// 2D y-pass, inverse: K (ny, nkx) -> W2 (ny, ld2).
 template <typename T>
-struct OpYInv2 {
+struct OpYInv2 : PlainRun<T, OpYInv2<T>> {
   StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int nkx, mx;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ifs, ofs;
+  int nkx, ld2;
   AxisMap ym;
   __device__ cpx<T> load(int f, int, int l, int p) const {
     if (ym.compact(p) < 0) return zero<T>();
-    return in[f][(size_t)p * nkx + l];
+    return in[f * ifs + p * nkx + l];
   }
   __device__ void store(int f, int, int l, int p, cpx<T> v) const {
-    out[f][(size_t)p * mx + l] = v;
+    out[f * ofs + p * ld2 + l] = v;
   }
 };
 
-// ns2d y-pass, inverse, from the vorticity: fields are
-// ux = i ky rot / k^2, uy = -i kx rot / k^2, d_x rot = i kx rot,
-// d_y rot = i ky rot (k^2 = 1e-14 at k = 0, the K2_not0 convention).
+// 2D y-pass, forward: W2 (ny, ld2) -> K (ny, nkx), kept modes only.
 template <typename T>
-struct OpYInv2NS {
+struct OpYFwd2 : PlainRun<T, OpYFwd2<T>> {
   StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int ny, nkx, mx;
-  AxisMap ym;
-  T dkx, dky;
-  __device__ cpx<T> load(int f, int, int l, int p) const {
-    if (ym.compact(p) < 0) return zero<T>();
-    const cpx<T> r = in[0][(size_t)p * nkx + l];
-    const T kx = T(l) * dkx;
-    const T ky = T(ksigned(p, ny)) * dky;
-    if (f == 0 || f == 1) {
-      T k2 = kx * kx + ky * ky;
-      if (k2 == T(0)) k2 = T(1e-14);
-      const T inv = T(1) / k2;
-      return f == 0 ? mul_i(scale(r, ky * inv)) : mul_i(scale(r, -kx * inv));
-    }
-    return mul_i(scale(r, f == 2 ? kx : ky));
-  }
-  __device__ void store(int f, int, int l, int p, cpx<T> v) const {
-    out[f][(size_t)p * mx + l] = v;
-  }
-};
-
-// 2D y-pass, forward: W2 (ny, mx) -> K (ny, nkx), kept modes only.
-template <typename T>
-struct OpYFwd2 {
-  StridedDesc<T> d;
-  const cpx<T>* in[8];
-  cpx<T>* out[8];
-  int nkx, mx;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ifs, ofs;
+  int nkx, ld2;
   AxisMap ym;
   const unsigned char* mask;
   __device__ cpx<T> load(int f, int, int l, int p) const {
-    return in[f][(size_t)p * mx + l];
+    return in[f * ifs + p * ld2 + l];
   }
   __device__ void store(int f, int, int l, int p, cpx<T> v) const {
     if (ym.compact(p) < 0) return;
-    const size_t idx = (size_t)p * nkx + l;
+    const int idx = p * nkx + l;
     if (mask && mask[idx]) v = zero<T>();
-    out[f][idx] = v;
+    out[f * ofs + idx] = v;
+  }
+};
+
+// ns3d z-pass, inverse, with the curl folded in.  omega = i k x v needs
+// i kz v only through d/dz, which is a transform of its own along z, while
+// kx and ky are constant on a z column and multiply after the transform:
+//   omega_x = i ky Vz - Dz[vy],  omega_y = Dz[vx] - i kx Vz,
+//   omega_z = i kx Vy - i ky Vx (+ f on the kx = ky = 0 column),
+// with V = iFFT_z(v) and Dz[a] = iFFT_z(i kz a).  Five transforms per column,
+// each input read from HBM once; outputs 0-2 are v, 3-5 omega.
+template <typename T>
+struct OpZInvCurl {
+  static constexpr int kEptDiv = 2;  // register stashes: half-size tiles
+  static constexpr bool kFieldPerBlock = false;
+  StridedDesc<T> d;
+  const cpx<T>* in;  // 3 fields
+  cpx<T>* out;       // 6 fields
+  long long ifs, ofs;
+  int ny, nz, nkx, ld1;
+  AxisMap ym, zm;
+  T dkx, dky, dkz;
+  int has_f;
+  T fcor;
+
+  template <int DIR, int EPT, int kT>
+  __device__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+    const int nlines = d.nlines;
+    const int iy = ym.full(o);
+    const T ky = T(ksigned(iy, ny)) * dky;
+    const int nyk = ym.nk();
+    cpx<T> sVz[EPT], sv[EPT], sVy[EPT];
+    auto ld = [&](int c, int l, int p) {
+      const int line = line0 + l;
+      if (line >= nlines || zm.compact(p) < 0) return zero<T>();
+      return in[c * ifs + (p * ny + iy) * nkx + line];
+    };
+    auto st = [&](int c, int l, int p, cpx<T> v) {
+      const int line = line0 + l;
+      if (line < nlines) out[c * ofs + (p * nyk + o) * ld1 + line] = v;
+    };
+    auto kz_of = [&](int p) { return T(ksigned(p, nz)) * dkz; };
+    // Vz
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+        [&](int e, int l, int p, cpx<T> v) {
+          sVz[e] = v;
+          st(2, l, p, v);
+        });
+    // omega_x = i ky Vz - Dz[vy]
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int e, int l, int p) {
+          const cpx<T> a = ld(1, l, p);
+          sv[e] = a;
+          return mul_i(scale(a, kz_of(p)));
+        },
+        [&](int e, int l, int p, cpx<T> v) {
+          st(3, l, p, mul_i(scale(sVz[e], ky)) - v);
+        });
+    // Vy
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int e, int, int) { return sv[e]; },
+        [&](int e, int l, int p, cpx<T> v) {
+          sVy[e] = v;
+          st(1, l, p, v);
+        });
+    // omega_y = Dz[vx] - i kx Vz
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int e, int l, int p) {
+          const cpx<T> a = ld(0, l, p);
+          sv[e] = a;
+          return mul_i(scale(a, kz_of(p)));
+        },
+        [&](int e, int l, int p, cpx<T> v) {
+          const T kx = T(line0 + l) * dkx;
+          st(4, l, p, v - mul_i(scale(sVz[e], kx)));
+        });
+    // Vx and omega_z = i kx Vy - i ky Vx (+ f)
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int e, int, int) { return sv[e]; },
+        [&](int e, int l, int p, cpx<T> v) {
+          const T kx = T(line0 + l) * dkx;
+          st(0, l, p, v);
+          cpx<T> w = mul_i(scale(sVy[e], kx) - scale(v, ky));
+          if (has_f && iy == 0 && line0 + l == 0) w.x += fcor;
+          st(5, l, p, w);
+        });
+  }
+};
+
+// ns2d y-pass, inverse, from the vorticity: one read of rot, four
+// transforms: ux = i ky rot / k^2, uy = -i kx rot / k^2, d_x rot = i kx rot,
+// d_y rot = i ky rot (k^2 = 1e-14 at k = 0, the K2_not0 convention).
+template <typename T>
+struct OpYInv2NS {
+  static constexpr int kEptDiv = 1;
+  static constexpr bool kFieldPerBlock = false;
+  StridedDesc<T> d;
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long ofs;
+  int ny, nkx, ld2;
+  AxisMap ym;
+  T dkx, dky;
+
+  template <int DIR, int EPT, int kT>
+  __device__ void run(cpx<T>* buf, const cpx<T>* tw, int, int line0) const {
+    const int nlines = d.nlines;
+    cpx<T> sr[EPT];
+    auto st = [&](int c, int l, int p, cpx<T> v) {
+      const int line = line0 + l;
+      if (line < nlines) out[c * ofs + p * ld2 + line] = v;
+    };
+    auto inv_k2 = [&](int l, int p) {
+      const T kx = T(line0 + l) * dkx, ky = T(ksigned(p, ny)) * dky;
+      T k2 = kx * kx + ky * ky;
+      if (k2 == T(0)) k2 = T(1e-14);
+      return T(1) / k2;
+    };
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int e, int l, int p) {
+          const int line = line0 + l;
+          cpx<T> r = zero<T>();
+          if (line < nlines && ym.compact(p) >= 0) r = in[p * nkx + line];
+          sr[e] = r;
+          const T ky = T(ksigned(p, ny)) * dky;
+          return mul_i(scale(r, ky * inv_k2(l, p)));
+        },
+        [&](int, int l, int p, cpx<T> v) { st(0, l, p, v); });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int e, int l, int p) {
+          const T kx = T(line0 + l) * dkx;
+          return mul_i(scale(sr[e], -kx * inv_k2(l, p)));
+        },
+        [&](int, int l, int p, cpx<T> v) { st(1, l, p, v); });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int e, int l, int) { return mul_i(scale(sr[e], T(line0 + l) * dkx)); },
+        [&](int, int l, int p, cpx<T> v) { st(2, l, p, v); });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int e, int, int p) { return mul_i(scale(sr[e], T(ksigned(p, ny)) * dky)); },
+        [&](int, int l, int p, cpx<T> v) { st(3, l, p, v); });
   }
 };
 
 // ---------------------------------------------------------------- rows
 template <typename T>
 struct XDesc {
-  int nx, M, logM;  // M = nx / 2 complex points per row
-  int RB;           // rows per workgroup
+  int nx, lognx, M, logM;  // M = nx / 2 complex points per row
+  int RB, logRB;           // rows per workgroup (power of two)
   int nrows;
   int nf_in, nf_out;
-  int mx_in, mx_out;  // kept kx on input (c2r) / output (r2c)
-  T scale;            // applied to the r2c output
-  const cpx<T>* tw;   // exp(-2 pi i m / nx), m < nx
+  int mx_in, ld_in;    // kept kx and row stride of the c2r input rows
+  int mx_out, ld_out;  // kept kx and row stride of the r2c output rows
+  long long ifs, ofs;  // field strides (elements)
+  T scale;             // applied to the r2c output
+  const cpx<T>* tw;    // exp(-2 pi i m / nx), m < nx (copied to LDS)
 };
 
 template <typename T>
@@ -269,10 +453,10 @@ __device__ __forceinline__ T wave_max(T v) {
 
 // Block-wide max of up to 3 values, then one atomicMax per value on the bit
 // pattern of a non-negative double (monotonic as unsigned 64-bit).
-template <typename T>
+template <typename T, int kT>
 __device__ void block_max_atomic(T m0, T m1, T m2, int nv,
                                  unsigned long long* dst) {
-  __shared__ T red[3][kThreads / 64];
+  __shared__ T red[3][kT / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   m0 = wave_max(m0);
   m1 = wave_max(m1);
@@ -285,181 +469,201 @@ __device__ void block_max_atomic(T m0, T m1, T m2, int nv,
   __syncthreads();
   if (threadIdx.x < nv) {
     T m = red[threadIdx.x][0];
-    for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
+    for (int w = 1; w < kT / 64; ++w) m = fmax(m, red[threadIdx.x][w]);
     const double md = (double)m;
     atomicMax(dst + threadIdx.x, (unsigned long long)__double_as_longlong(md));
   }
   __syncthreads();
 }
 
-template <typename T, int EPT, class Op>
-__global__ __launch_bounds__(kThreads) void k_xpass(Op op) {
+template <typename T, int EPT, int kT, class Op>
+__global__ __launch_bounds__(kT) void k_xpass(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
-  T* bufr = reinterpret_cast<T*>(smem_raw);
   const XDesc<T>& d = op.d;
-  const int M = d.M, logM = d.logM, nx = d.nx, RB = d.RB;
-  const int row0 = blockIdx.x * RB;
+  cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* buf = s_tw + d.nx;
+  T* bufr = reinterpret_cast<T*>(buf);
+  const int M = d.M, logM = d.logM, nx = d.nx;
+  const int H = M >> 1, logH = logM - 1;
+  const int row0 = blockIdx.x << d.logRB;
   const int tid = threadIdx.x;
+  for (int i = tid; i < nx; i += kT) s_tw[i] = d.tw[i];
+  __syncthreads();
 
   if constexpr (Op::kInv) {
-    // c2r pre-processing: Z[k] = (X[k] + conj X[M-k])
-    //                          + i conj(W^k) (X[k] - conj X[M-k]),
-    // W = exp(-2 pi i / nx); only the real parts of X[0] and X[M] are used.
-    const int n_in = d.nf_in * RB * M;
-    for (int idx = tid; idx < n_in; idx += kThreads) {
-      const int k = idx & (M - 1);
-      const int lr = idx >> logM;
-      const int f = lr / RB;
-      const int r = lr - f * RB;
-      const int row = row0 + r;
-      cpx<T> Xk = zero<T>(), Xm = zero<T>();
+    // c2r pre-processing, one thread per pair (k, M - k):
+    // Z[k] = (X[k] + conj X[M-k]) + i conj(W^k) (X[k] - conj X[M-k]),
+    // W = exp(-2 pi i / nx); only the real parts of X[0] and X[M] are used;
+    // the k = 0 thread also does the self-paired k = M/2.
+    const int n_in = (d.nf_in << d.logRB) << logH;
+    for (int idx = tid; idx < n_in; idx += kT) {
+      const int k = idx & (H - 1);
+      const int lr = idx >> logH;
+      const int f = lr >> d.logRB;
+      const int row = row0 + (lr & (d.RB - 1));
+      cpx<T>* Z = buf + (lr << logM);
+      cpx<T> Xk = zero<T>(), Xm = zero<T>(), Xh = zero<T>();
       if (row < d.nrows) {
-        const cpx<T>* src = op.in[f] + (size_t)row * d.mx_in;
+        const cpx<T>* src = op.in + f * d.ifs + (long long)row * d.ld_in;
         if (k < d.mx_in) Xk = src[k];
         if (M - k < d.mx_in) Xm = src[M - k];
+        if (k == 0 && H < d.mx_in) Xh = src[H];
       }
       if (k == 0) {
         Xk.y = T(0);
         Xm.y = T(0);
       }
-      const cpx<T> a = Xk + conj(Xm);
-      const cpx<T> t = conj(d.tw[k]) * (Xk - conj(Xm));
-      buf[idx] = a + mul_i(t);
+      const cpx<T> w = s_tw[k];
+      const cpx<T> dk = Xk - conj(Xm), dm = Xm - conj(Xk);
+      Z[k] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
+      if (k == 0) {
+        const cpx<T> dh = Xh - conj(Xh);
+        Z[H] = (Xh + conj(Xh)) + mul_i(conj(s_tw[H]) * dh);
+      } else {
+        // conj(W^(M-k)) = -W^k
+        Z[M - k] = (Xm + conj(Xk)) - mul_i(w * dm);
+      }
     }
     __syncthreads();
-    lds_fft<T, +1, EPT>(buf, logM, d.nf_in * RB, false, 1, M, d.tw, 1);
+    lds_fft_rows<T, +1, EPT, kT>(buf, logM, d.nf_in << d.logRB, M, s_tw, 1);
   } else {
-    // load real rows
-    const int n_in = d.nf_in * RB * nx;
-    for (int idx = tid; idx < n_in; idx += kThreads) {
+    const int n_in = (d.nf_in << d.logRB) << d.lognx;
+    for (int idx = tid; idx < n_in; idx += kT) {
       const int x = idx & (nx - 1);
-      const int lr = idx / nx;
-      const int f = lr / RB;
-      const int r = lr - f * RB;
-      const int row = row0 + r;
-      bufr[idx] = row < d.nrows ? op.in[f][(size_t)row * nx + x] : T(0);
+      const int lr = idx >> d.lognx;
+      const int f = lr >> d.logRB;
+      const int row = row0 + (lr & (d.RB - 1));
+      bufr[idx] = row < d.nrows ? op.inr[f * d.ifs + (long long)row * nx + x] : T(0);
     }
     __syncthreads();
   }
 
   if constexpr (Op::kStoreReal) {
-    const int n_out = d.nf_in * RB * nx;
-    for (int idx = tid; idx < n_out; idx += kThreads) {
+    const int n_out = (d.nf_in << d.logRB) << d.lognx;
+    for (int idx = tid; idx < n_out; idx += kT) {
       const int x = idx & (nx - 1);
-      const int lr = idx / nx;
-      const int f = lr / RB;
-      const int r = lr - f * RB;
-      const int row = row0 + r;
-      if (row < d.nrows) op.outr[f][(size_t)row * nx + x] = bufr[idx];
+      const int lr = idx >> d.lognx;
+      const int f = lr >> d.logRB;
+      const int row = row0 + (lr & (d.RB - 1));
+      if (row < d.nrows) op.outr[f * d.ofs + (long long)row * nx + x] = bufr[idx];
     }
     return;
   } else {
     op.physical(bufr, row0);
     __syncthreads();
-    lds_fft<T, -1, EPT>(buf, logM, d.nf_out * RB, false, 1, M, d.tw, 1);
-    // r2c post-processing: X[k] = E[k] + W^k O[k] with
-    // E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2.
-    const int n_out = d.nf_out * RB * d.mx_out;
-    for (int idx = tid; idx < n_out; idx += kThreads) {
-      const int k = idx % d.mx_out;
-      const int lr = idx / d.mx_out;
-      const int f = lr / RB;
-      const int r = lr - f * RB;
-      const int row = row0 + r;
+    lds_fft_rows<T, -1, EPT, kT>(buf, logM, d.nf_out << d.logRB, M, s_tw, 1);
+    // r2c post-processing, one thread per pair (k, M - k):
+    // E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2,
+    // X[k] = E + W^k O, X[M-k] = conj(E - W^k O); k = 0 also does M/2.
+    const int n_out = (d.nf_out << d.logRB) << logH;
+    for (int idx = tid; idx < n_out; idx += kT) {
+      const int k = idx & (H - 1);
+      const int lr = idx >> logH;
+      const int f = lr >> d.logRB;
+      const int row = row0 + (lr & (d.RB - 1));
       if (row >= d.nrows) continue;
-      const cpx<T>* Z = buf + lr * M;
-      const cpx<T> Zk = Z[k & (M - 1)];
-      const cpx<T> Zm = conj(Z[(M - k) & (M - 1)]);
-      const cpx<T> E = scale(Zk + Zm, T(0.5));
-      const cpx<T> dd = Zk - Zm;
-      const cpx<T> O = mk<T>(dd.y * T(0.5), -dd.x * T(0.5));
-      const cpx<T> X = scale(E + d.tw[k] * O, d.scale);
-      op.out[f][(size_t)row * d.mx_out + k] = X;
+      const cpx<T>* Z = buf + (lr << logM);
+      cpx<T>* dst = op.out + f * d.ofs + (long long)row * d.ld_out;
+      const T h = T(0.5);
+      {
+        const cpx<T> Zk = Z[k], Zm = conj(Z[(M - k) & (M - 1)]);
+        const cpx<T> E = scale(Zk + Zm, h), dd = Zk - Zm;
+        const cpx<T> P = s_tw[k] * mk<T>(dd.y * h, -dd.x * h);
+        if (k < d.mx_out) dst[k] = scale(E + P, d.scale);
+        if (M - k < d.mx_out) dst[M - k] = scale(conj(E - P), d.scale);
+      }
+      if (k == 0 && H < d.mx_out) {
+        const cpx<T> Zk = Z[H], Zm = conj(Z[H]);
+        const cpx<T> E = scale(Zk + Zm, h), dd = Zk - Zm;
+        const cpx<T> P = s_tw[H] * mk<T>(dd.y * h, -dd.x * h);
+        dst[H] = scale(E + P, d.scale);
+      }
     }
   }
 }
 
-// plain c2r rows: W2 rows (mx_in) -> X rows
+// plain c2r rows: W2 rows -> X rows
 template <typename T>
 struct OpXC2R {
   static constexpr bool kInv = true, kStoreReal = true;
   XDesc<T> d;
-  const cpx<T>* in[8];
-  T* outr[8];
-  cpx<T>* out[8];
+  const cpx<T>* in;
+  const T* inr;
+  T* outr;
+  cpx<T>* out;
   __device__ void physical(T*, int) const {}
 };
 
-// plain r2c rows: X rows -> W2 rows (mx_out), scaled by 1/(number of points)
+// plain r2c rows: X rows -> W2 rows, scaled by 1/(number of points)
 template <typename T>
 struct OpXR2C {
   static constexpr bool kInv = false, kStoreReal = false;
   XDesc<T> d;
-  const T* in[8];
-  T* outr[8];
-  cpx<T>* out[8];
+  const cpx<T>* in;
+  const T* inr;
+  T* outr;
+  cpx<T>* out;
   __device__ void physical(T*, int) const {}
 };
 
 // ns3d rows: (v, omega) -> v x omega, optional CFL maxima of |v_i|.
-template <typename T>
+template <typename T, int kT>
 struct OpXNS3D {
   static constexpr bool kInv = true, kStoreReal = false;
   XDesc<T> d;
-  const cpx<T>* in[8];
-  T* outr[8];
-  cpx<T>* out[8];
+  const cpx<T>* in;
+  const T* inr;
+  T* outr;
+  cpx<T>* out;
   unsigned long long* maxima;  // 3 slots or null
   __device__ void physical(T* b, int row0) const {
-    const int nx = d.nx, RB = d.RB;
+    const int nx = d.nx;
+    const int n = d.RB << d.lognx;
+    const int s = n;  // reals per field
     T m0 = 0, m1 = 0, m2 = 0;
-    for (int idx = threadIdx.x; idx < RB * nx; idx += kThreads) {
-      const int r = idx / nx;
-      const int x = idx - r * nx;
-      const size_t s = (size_t)RB * nx;
-      T* p = b + (size_t)r * nx + x;
+    for (int idx = threadIdx.x; idx < n; idx += kT) {
+      T* p = b + idx;
       const T vx = p[0], vy = p[s], vz = p[2 * s];
       const T wx = p[3 * s], wy = p[4 * s], wz = p[5 * s];
       p[0] = vy * wz - vz * wy;
       p[s] = vz * wx - vx * wz;
       p[2 * s] = vx * wy - vy * wx;
-      if (row0 + r < d.nrows) {
+      if (row0 + (idx >> d.lognx) < d.nrows) {
         m0 = fmax(m0, fabs(vx));
         m1 = fmax(m1, fabs(vy));
         m2 = fmax(m2, fabs(vz));
       }
     }
-    if (maxima) block_max_atomic<T>(m0, m1, m2, 3, maxima);
+    (void)nx;
+    if (maxima) block_max_atomic<T, kT>(m0, m1, m2, 3, maxima);
   }
 };
 
 // ns2d rows: (ux, uy, d_x rot, d_y rot) -> -ux d_x rot - uy (d_y rot + beta)
-template <typename T>
+template <typename T, int kT>
 struct OpXNS2D {
   static constexpr bool kInv = true, kStoreReal = false;
   XDesc<T> d;
-  const cpx<T>* in[8];
-  T* outr[8];
-  cpx<T>* out[8];
+  const cpx<T>* in;
+  const T* inr;
+  T* outr;
+  cpx<T>* out;
   unsigned long long* maxima;  // 2 slots or null
   T beta;
   __device__ void physical(T* b, int row0) const {
-    const int nx = d.nx, RB = d.RB;
+    const int n = d.RB << d.lognx;
+    const int s = n;
     T m0 = 0, m1 = 0;
-    for (int idx = threadIdx.x; idx < RB * nx; idx += kThreads) {
-      const int r = idx / nx;
-      const int x = idx - r * nx;
-      const size_t s = (size_t)RB * nx;
-      T* p = b + (size_t)r * nx + x;
+    for (int idx = threadIdx.x; idx < n; idx += kT) {
+      T* p = b + idx;
       const T ux = p[0], uy = p[s], px = p[2 * s], py = p[3 * s];
       p[0] = beta == T(0) ? -ux * px - uy * py : -ux * px - uy * (py + beta);
-      if (row0 + r < d.nrows) {
+      if (row0 + (idx >> d.lognx) < d.nrows) {
         m0 = fmax(m0, fabs(ux));
         m1 = fmax(m1, fabs(uy));
       }
     }
-    if (maxima) block_max_atomic<T>(m0, m1, T(0), 2, maxima);
+    if (maxima) block_max_atomic<T, kT>(m0, m1, T(0), 2, maxima);
   }
 };
 

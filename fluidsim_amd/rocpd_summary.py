@@ -1,0 +1,80 @@
+"""Summarise rocprofv3 result databases (rocpd SQLite) as text.
+
+    python -m fluidsim_amd.rocpd_summary kernels DB [DB_PMC ...]
+
+``kernels``: per kernel name, calls, average / total duration (from the
+kernel trace); with PMC databases, the average counter value per dispatch of
+each kernel is added (FETCH_SIZE / WRITE_SIZE are in KiB per dispatch).
+"""
+
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def short_name(name):
+    """Readable kernel name: template arguments of the pass kernels kept."""
+    m = re.search(r"fsx::(k_\w+)<([^>]*(?:<[^>]*>)?[^>]*)>", name)
+    if m:
+        args = m.group(2)
+        args = args.replace("fsx::", "").replace("float", "f32").replace("double", "f64")
+        return f"{m.group(1)}<{args}>"
+    m = re.search(r"fsx::(k_\w+)", name)
+    if m:
+        return m.group(1)
+    return name[:80]
+
+
+def kernel_table(db):
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, duration from kernels").fetchall()
+    agg = defaultdict(list)
+    for name, dur in rows:
+        agg[short_name(name)].append(dur)
+    return agg
+
+
+def pmc_table(db):
+    con = sqlite3.connect(db)
+    try:
+        rows = con.execute(
+            "select k.name, p.counter_name, p.value from pmc_events p "
+            "join kernels k on p.event_id = k.event_id"
+        ).fetchall()
+    except sqlite3.Error:
+        cols = [r[1] for r in con.execute("pragma table_info('counters_collection')")]
+        name_col = "kernel_name" if "kernel_name" in cols else "name"
+        rows = con.execute(
+            f"select {name_col}, counter_name, value from counters_collection"
+        ).fetchall()
+    agg = defaultdict(lambda: defaultdict(list))
+    for name, counter, value in rows:
+        agg[short_name(name)][counter].append(value)
+    return agg
+
+
+def main(argv):
+    if len(argv) < 2 or argv[0] != "kernels":
+        print(__doc__)
+        return 2
+    kt = kernel_table(argv[1])
+    pmcs = [pmc_table(db) for db in argv[2:]]
+    total = sum(sum(v) for v in kt.values())
+    counters = sorted({c for p in pmcs for k in p.values() for c in k})
+    hdr = f"{'kernel':60s} {'calls':>6s} {'avg_us':>10s} {'total_ms':>10s} {'pct':>6s}"
+    for c in counters:
+        hdr += f" {c + '/disp':>18s}"
+    print(hdr)
+    for name, durs in sorted(kt.items(), key=lambda kv: -sum(kv[1])):
+        line = (f"{name[:60]:60s} {len(durs):6d} {sum(durs) / len(durs) / 1e3:10.1f} "
+                f"{sum(durs) / 1e6:10.2f} {100 * sum(durs) / total:6.1f}")
+        for c in counters:
+            vals = [v for p in pmcs for v in p.get(name, {}).get(c, [])]
+            line += f" {(sum(vals) / len(vals)) if vals else float('nan'):18.1f}"
+        print(line)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
