@@ -17,6 +17,7 @@ the algorithmic bytes B_alg = T (b_X + b_K) + R b_K per RK4 step
 """
 
 import argparse
+import contextlib
 import json
 import math
 import os
@@ -145,7 +146,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     print(f"[bench] building ns3d {n}^3 {args.precision}", file=sys.stderr, flush=True)
-    sim = make_sim(n, args.precision)
+    # the solver's own progress messages go to stderr: stdout is the JSON line
+    with contextlib.redirect_stdout(sys.stderr):
+        sim = make_sim(n, args.precision)
     ts = sim.time_stepping
     plan = sim.oper.plan.handle
     print(f"[bench] dt = {ts.deltat:.6g}; warmup {args.warmup}", file=sys.stderr, flush=True)

@@ -12,7 +12,9 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfluidsim_amd.so")
+# FLUIDSIM_AMD_LIB selects another build of the engine library (e.g. a
+# variant compiled with different tile sizes).
+LIB_PATH = os.environ.get("FLUIDSIM_AMD_LIB") or os.path.join(HERE, "libfluidsim_amd.so")
 
 FSX_ERR_ARG = -1
 FSX_ERR_HIP = -2
@@ -44,6 +46,19 @@ SIGNATURES = {
         P, c_int, c_int, c_int, P, P, P, P, c_int, c_int,
         ctypes.POINTER(c_double), P,
     ],
+    "fsx_ns2d_rk_stage": [
+        P, c_int, c_int, P, P, P, c_double, ctypes.POINTER(c_double), c_double,
+        c_double, P,
+    ],
+    "fsx_ns3d_rk_stage": [
+        P, c_int, c_int, P, P, P, c_int, c_double, c_int, c_int,
+        ctypes.POINTER(c_double), c_double, c_double, P,
+    ],
+    "fsx_strat_rk_stage": [
+        P, c_int, c_int, P, P, P, c_int, c_double, c_double, c_int, c_int,
+        ctypes.POINTER(c_double), c_double, c_double, P,
+    ],
+    "fsx_strat_tendency": [P, P, P, c_int, c_double, c_double, c_int, c_int, c_int, P],
     "fsx_project": [P, P, P, P, c_int, P],
     "fsx_dealias": [P, P, c_int, c_int, P],
     "fsx_set_dt": [P, c_double, P],

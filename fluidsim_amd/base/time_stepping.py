@@ -211,6 +211,11 @@ class TimeSteppingPseudoSpectral:
         sc = self._read_scalars()
         self.deltat = self._dt_device = sc[4]
 
+    # fused stages: the tendency and the Runge-Kutta algebra of a stage in
+    # one pass sequence (the tendency never goes to HBM); False selects the
+    # separate tendency + stage kernels (same results, used to cross-check)
+    fused = True
+
     def _time_step_RK(self):
         p = self.params.time_stepping
         sim = self.sim
@@ -219,11 +224,18 @@ class TimeSteppingPseudoSpectral:
         S = sim.state.state_spect.data
         tmp = self._state_spect_tmp.data
         t1 = self._state_spect_tmp1.data
+        self._push_dt()
+        if self.fused and hasattr(sim, "_engine_rk_stage"):
+            cfl = float(self.CFL) if p.USE_CFL else 0.0
+            for stage in range(self._scheme):
+                sim._engine_rk_stage(self._scheme, stage, S, tmp, t1, self._visc,
+                                     cfl if stage == 0 else 0.0,
+                                     float(self.deltat_max))
+            return
         N = self._tendencies.data
         nvar = S.shape[0]
         proj = sim._engine_projection
         novz = int(getattr(sim, "no_vz_kz0", False))
-        self._push_dt()
         if p.USE_CFL:
             _lib.call("fsx_reset_maxima", plan, s)
         sim._engine_tendency(S, N, want_maxima=bool(p.USE_CFL))

@@ -142,7 +142,14 @@ int tm_flush(fsx_plan_s* p) {
   return 0;
 }
 
-constexpr int kThr = 512;   // threads per workgroup of the pass kernels
+constexpr int kThr = 512;   // threads per workgroup of the strided passes
+#ifndef FSX_XPASS_THREADS
+#define FSX_XPASS_THREADS 512
+#endif
+#ifndef FSX_XPASS_EPT_DIV
+#define FSX_XPASS_EPT_DIV 2
+#endif
+constexpr int kXThr = FSX_XPASS_THREADS;  // threads per workgroup of the row pass
 constexpr int kEPT32 = 16;  // 8192 complex64 per tile  = 64 KiB of LDS
 constexpr int kEPT64 = 8;   // 4096 complex128 per tile = 64 KiB of LDS
 constexpr int kMinBlocks = 1024;  // shrink tiles of small problems below this
@@ -208,7 +215,7 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   const long long nchunks = (nlines + L - 1) / L;
   const long long blocks = nchunks * nouter;
   if (blocks > 0x7fffffffLL) return fail(FSX_ERR_ARG, "grid too large");
-  const size_t lds = ((size_t)tile + N) * sizeof(cpx<T>);
+  const size_t lds = ((size_t)tile + tile / 32 + N) * sizeof(cpx<T>);
   auto kern = k_strided<T, DIR, E, kThr, Op>;
   static size_t lds_set = 0;
   if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
@@ -229,8 +236,8 @@ template <typename T, class Op>
 int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
                  int ld_in, int mx_out, int ld_out, long long ifs,
                  long long ofs, T scl, const void* tw, hipStream_t s) {
-  constexpr int E = ept<T>();
-  const int tile = E * kThr;
+  constexpr int E = ept<T>() / FSX_XPASS_EPT_DIV >= 8 ? ept<T>() / FSX_XPASS_EPT_DIV : 8;
+  const int tile = E * kXThr;
   const int M = nx / 2;
   const int nfm = nf_in > nf_out ? nf_in : nf_out;
   if (nfm * M > tile) return fail(FSX_ERR_ARG, "row pass exceeds the tile");
@@ -256,8 +263,8 @@ int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
   op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
   if (nrows <= 0) return 0;
   const long long blocks = (nrows + RB - 1) / RB;
-  const size_t lds = ((size_t)tile + nx) * sizeof(cpx<T>);
-  auto kern = k_xpass<T, E, kThr, Op>;
+  const size_t lds = ((size_t)tile + tile / 32 + nfm * RB + nx) * sizeof(cpx<T>);
+  auto kern = k_xpass<T, E, kXThr, Op>;
   static size_t lds_set = 0;
   if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
   if (lds > lds_set) {
@@ -267,7 +274,7 @@ int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
   }
   int rc = tm_pre(s);
   if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThr), lds, s, op);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kXThr), lds, s, op);
   FSX_HIP(hipGetLastError());
   return tm_post(s);
 }
@@ -464,9 +471,11 @@ int do_fft(fsx_plan_s* p, const void* xin, void* kout, int nfields,
   return 0;
 }
 
+// ns2d: y-pass inverse (u, v, grad rot) and row pass (products, r2c) into
+// W2 field 0.
 template <typename T>
-int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
-            int want_maxima, hipStream_t s) {
+int ns2d_front(fsx_plan_s* p, const void* rot, double beta, int want_maxima,
+               hipStream_t s) {
   const Geo g = geo<T>(p, true);
   OpYInv2NS<T> op;
   op.in = reinterpret_cast<const cpx<T>*>(rot);
@@ -481,7 +490,7 @@ int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
   note(K_YINV, (double)g.mx * (g.ym.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
   int rc = launch_strided<T, +1>(op, p->ny, 1, g.mx, 4, p->tw[1], s);
   if (rc) return rc;
-  OpXNS2D<T, kThr> ox;
+  OpXNS2D<T, kXThr> ox;
   ox.in = W2<T>(p);
   ox.out = W2<T>(p);
   ox.maxima =
@@ -489,17 +498,27 @@ int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
   ox.beta = (T)beta;
   const double npts = (double)p->nx * p->ny;
   note(K_XPASS, 5.0 * p->ny * g.mx * sizeof(cpx<T>));
-  rc = launch_xpass<T>(ox, p->nx, p->ny, 4, 1, g.mx, g.ld2, g.mx, g.ld2, g.fs2,
-                       g.fs2, T(1.0 / npts), p->tw[0], s);
+  return launch_xpass<T>(ox, p->nx, p->ny, 4, 1, g.mx, g.ld2, g.mx, g.ld2,
+                         g.fs2, g.fs2, T(1.0 / npts), p->tw[0], s);
+}
+
+template <typename T>
+int do_ns2d(fsx_plan_s* p, const void* rot, void* out, double beta,
+            int want_maxima, hipStream_t s) {
+  int rc = ns2d_front<T>(p, rot, beta, want_maxima, s);
   if (rc) return rc;
   return forward_from_w2<T>(p, reinterpret_cast<cpx<T>*>(out), 0, 1, true, s);
 }
 
+// ns3d (strat = 0): z-pass with curl, y-pass, row pass (v x omega, r2c)
+// into W2 fields 0-2.  ns3d.strat (strat = 1): v, omega and b through the
+// z- and y-passes, row pass producing v x omega and v b into W2 fields 0-5.
 template <typename T>
-int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
-            int want_maxima, hipStream_t s) {
+int ns3d_front(fsx_plan_s* p, const void* v, int has_f, double fcor,
+               int want_maxima, hipStream_t s, int strat = 0) {
   const Geo g = geo<T>(p, true);
   const long long ke = (long long)kelems(p);
+  const int nin = strat ? 7 : 6, nout = strat ? 6 : 3;
   OpZInvCurl<T> op;
   op.in = reinterpret_cast<const cpx<T>*>(v);
   op.out = W1<T>(p);
@@ -516,9 +535,10 @@ int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
   op.dkz = (T)p->dkz;
   op.has_f = has_f;
   op.fcor = (T)fcor;
-  note(K_ZINV, (double)g.ym.nk() * g.mx * (3.0 * g.zm.nk() + 6.0 * p->nz) *
-                   sizeof(cpx<T>));
-  int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, 6, p->tw[2], s);
+  op.with_b = strat;
+  note(K_ZINV, (double)g.ym.nk() * g.mx *
+                   ((nin - 3.0) * g.zm.nk() + (double)nin * p->nz) * sizeof(cpx<T>));
+  int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, nin, p->tw[2], s);
   if (rc) return rc;
   OpYInv3<T> oy;
   oy.in = W1<T>(p);
@@ -529,33 +549,50 @@ int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
   oy.ld1 = g.ld1;
   oy.ld2 = g.ld2;
   oy.ym = g.ym;
-  note(K_YINV, 6.0 * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
-  rc = launch_strided<T, +1>(oy, p->ny, p->nz, g.mx, 6, p->tw[1], s);
+  note(K_YINV, (double)nin * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+  rc = launch_strided<T, +1>(oy, p->ny, p->nz, g.mx, nin, p->tw[1], s);
   if (rc) return rc;
-  OpXNS3D<T, kThr> ox;
-  ox.in = W2<T>(p);
-  ox.out = W2<T>(p);
-  ox.maxima =
+  unsigned long long* mx =
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
   const double npts = (double)p->nx * p->ny * p->nz;
-  note(K_XPASS, 9.0 * nrows(p) * g.mx * sizeof(cpx<T>));
-  rc = launch_xpass<T>(ox, p->nx, nrows(p), 6, 3, g.mx, g.ld2, g.mx, g.ld2,
-                       g.fs2, g.fs2, T(1.0 / npts), p->tw[0], s);
-  if (rc) return rc;
-  return forward_from_w2<T>(p, reinterpret_cast<cpx<T>*>(out), ke, 3, true, s);
+  note(K_XPASS, (double)(nin + nout) * nrows(p) * g.mx * sizeof(cpx<T>));
+  if (strat) {
+    OpXStrat<T, kXThr> ox;
+    ox.in = W2<T>(p);
+    ox.out = W2<T>(p);
+    ox.maxima = mx;
+    return launch_xpass<T>(ox, p->nx, nrows(p), nin, nout, g.mx, g.ld2, g.mx,
+                           g.ld2, g.fs2, g.fs2, T(1.0 / npts), p->tw[0], s);
+  }
+  OpXNS3D<T, kXThr> ox;
+  ox.in = W2<T>(p);
+  ox.out = W2<T>(p);
+  ox.maxima = mx;
+  return launch_xpass<T>(ox, p->nx, nrows(p), nin, nout, g.mx, g.ld2, g.mx,
+                         g.ld2, g.fs2, g.fs2, T(1.0 / npts), p->tw[0], s);
 }
 
 template <typename T>
-int do_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S, void* tmp,
-          void* t1, void* N, int project, int no_vz_kz0, const double* visc,
-          hipStream_t s) {
+int do_ns3d(fsx_plan_s* p, const void* v, void* out, int has_f, double fcor,
+            int want_maxima, hipStream_t s) {
+  int rc = ns3d_front<T>(p, v, has_f, fcor, want_maxima, s);
+  if (rc) return rc;
+  return forward_from_w2<T>(p, reinterpret_cast<cpx<T>*>(out),
+                            (long long)kelems(p), 3, true, s);
+}
+
+template <typename T>
+RKArgs<T> make_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S,
+                  void* tmp, void* t1, void* N, int project, int no_vz_kz0,
+                  const double* visc) {
   RKArgs<T> a;
   const size_t ke = kelems(p);
-  for (int v = 0; v < nvar; ++v) {
-    a.S[v] = reinterpret_cast<cpx<T>*>(S) + v * ke;
-    a.tmp[v] = tmp ? reinterpret_cast<cpx<T>*>(tmp) + v * ke : nullptr;
-    a.t1[v] = reinterpret_cast<cpx<T>*>(t1) + v * ke;
-    a.Nt[v] = reinterpret_cast<cpx<T>*>(N) + v * ke;
+  for (int v = 0; v < 4; ++v) {
+    const bool on = v < nvar;
+    a.S[v] = on ? reinterpret_cast<cpx<T>*>(S) + v * ke : nullptr;
+    a.tmp[v] = on && tmp ? reinterpret_cast<cpx<T>*>(tmp) + v * ke : nullptr;
+    a.t1[v] = on ? reinterpret_cast<cpx<T>*>(t1) + v * ke : nullptr;
+    a.Nt[v] = on && N ? reinterpret_cast<cpx<T>*>(N) + v * ke : nullptr;
   }
   a.nvar = nvar;
   a.scheme = scheme;
@@ -569,15 +606,125 @@ int do_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S, void* tmp,
   a.visc.num4 = visc[3];
   a.box = make_box(p, true);
   a.nanflag = reinterpret_cast<unsigned int*>(p->d_scal + 5);
-  const size_t n = (size_t)(p->dim == 3 ? p->zmP.nk() : 1) * p->ymP.nk() * p->mxP;
-  const int nstream = scheme == 4 ? (stage == 0 ? 4 : (stage == 3 ? 3 : 5))
-                                  : (stage == 0 ? 3 : 3);
-  note(K_RK, (double)n * nvar * nstream * 2 * sizeof(T));
+  return a;
+}
+
+// algorithmic bytes of the state streams of one Runge-Kutta stage
+double rk_bytes(const fsx_plan_s* p, int scheme, int stage, int nvar, int csz) {
+  const double kept = (double)(p->dim == 3 ? p->zmP.nk() : 1) * p->ymP.nk() * p->mxP;
+  const int streams = scheme == 4 ? (stage == 0 ? 3 : (stage == 3 ? 2 : 4))
+                                  : (stage == 0 ? 2 : 2);
+  return kept * nvar * streams * csz;
+}
+
+template <typename T>
+int do_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S, void* tmp,
+          void* t1, void* N, int project, int no_vz_kz0, const double* visc,
+          hipStream_t s) {
+  RKArgs<T> a = make_rk<T>(p, scheme, stage, nvar, S, tmp, t1, N, project,
+                           no_vz_kz0, visc);
+  const double kept = (double)(p->dim == 3 ? p->zmP.nk() : 1) * p->ymP.nk() * p->mxP;
+  note(K_RK, rk_bytes(p, scheme, stage, nvar, sizeof(cpx<T>)) +
+                 kept * nvar * sizeof(cpx<T>));
   int rc = tm_pre(s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_rk_stage<T>, dim3(grid_for(n)), dim3(256), 0, s, a);
+  const int nrow = (p->dim == 3 ? p->zmP.nk() : 1) * p->ymP.nk();
+  unsigned blocks = (unsigned)((nrow + 3) / 4);
+  if (blocks > 16384) blocks = 16384;
+  if (nvar == 1)
+    hipLaunchKernelGGL((k_rk_stage<T, 1>), dim3(blocks), dim3(256), 0, s, a);
+  else if (nvar == 2)
+    hipLaunchKernelGGL((k_rk_stage<T, 2>), dim3(blocks), dim3(256), 0, s, a);
+  else if (nvar == 3)
+    hipLaunchKernelGGL((k_rk_stage<T, 3>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_rk_stage<T, 4>), dim3(blocks), dim3(256), 0, s, a);
   FSX_HIP(hipGetLastError());
   return tm_post(s);
+}
+
+int cfl_begin(fsx_plan_s* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s,
+                     reinterpret_cast<unsigned long long*>(p->d_scal), 4, 0ull);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int cfl_end(fsx_plan_s* p, int ncomp, double cfl, double dt_max, hipStream_t s) {
+  const double dx = p->Lx / p->nx, dy = p->Ly / p->ny,
+               dz = p->dim == 3 ? p->Lz / p->nz : 1.0;
+  hipLaunchKernelGGL(k_cfl_update, dim3(1), dim3(64), 0, s,
+                     reinterpret_cast<const unsigned long long*>(p->d_scal),
+                     ncomp, dx, dy, dz, cfl, dt_max, p->d_scal + 4);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+// Fused ns2d Runge-Kutta stage: tendency of S (stage 0) or t1, optional CFL
+// update of the device dt at stage 0, forward y-pass with the stage algebra.
+template <typename T>
+int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
+               void* t1, double beta, const double* visc, double cfl,
+               double dt_max, hipStream_t s) {
+  const bool use_cfl = cfl > 0 && stage == 0;
+  int rc = 0;
+  if (use_cfl && (rc = cfl_begin(p, s))) return rc;
+  rc = ns2d_front<T>(p, stage == 0 ? S : t1, beta, use_cfl, s);
+  if (rc) return rc;
+  if (use_cfl && (rc = cfl_end(p, 2, cfl, dt_max, s))) return rc;
+  const Geo g = geo<T>(p, true);
+  OpYFwd2RK<T> op;
+  op.in = W2<T>(p);
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.ld2 = g.ld2;
+  op.ym = g.ym;
+  op.mask = p->d_mask;
+  op.rk = make_rk<T>(p, scheme, stage, 1, S, tmp, t1, nullptr, 0, 0, visc);
+  note(K_ZFWD, (double)g.mx * p->ny * sizeof(cpx<T>) +
+                   rk_bytes(p, scheme, stage, 1, sizeof(cpx<T>)));
+  return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s);
+}
+
+template <typename T>
+int do_ns3d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
+               void* t1, int has_f, double fcor, int project, int no_vz_kz0,
+               const double* visc, double cfl, double dt_max, hipStream_t s) {
+  const bool use_cfl = cfl > 0 && stage == 0;
+  int rc = 0;
+  if (use_cfl && (rc = cfl_begin(p, s))) return rc;
+  rc = ns3d_front<T>(p, stage == 0 ? S : t1, has_f, fcor, use_cfl, s);
+  if (rc) return rc;
+  const Geo g = geo<T>(p, true);
+  {
+    OpYFwd3<T> op;
+    op.in = W2<T>(p);
+    op.out = W1<T>(p);
+    op.ifs = g.fs2;
+    op.ofs = g.fs1;
+    op.ny = p->ny;
+    op.ld1 = g.ld1;
+    op.ld2 = g.ld2;
+    op.ym = g.ym;
+    note(K_YFWD, 3.0 * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+    rc = launch_strided<T, -1>(op, p->ny, p->nz, g.mx, 3, p->tw[1], s);
+    if (rc) return rc;
+  }
+  if (use_cfl && (rc = cfl_end(p, 3, cfl, dt_max, s))) return rc;
+  OpZFwdRK3<T> op;
+  op.in = W1<T>(p);
+  op.ifs = g.fs1;
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.ld1 = g.ld1;
+  op.ym = g.ym;
+  op.zm = g.zm;
+  op.mask = p->d_mask;
+  op.rk = make_rk<T>(p, scheme, stage, 3, S, tmp, t1, nullptr, project, no_vz_kz0,
+                     visc);
+  note(K_ZFWD, 3.0 * g.ym.nk() * g.mx * p->nz * sizeof(cpx<T>) +
+                   rk_bytes(p, scheme, stage, 3, sizeof(cpx<T>)));
+  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, 3, p->tw[2], s);
 }
 
 template <typename T>
@@ -598,6 +745,60 @@ int do_sums(fsx_plan_s* p, int kind, const void* f0, const void* f1,
   FSX_HIP(hipMemcpyAsync(host_out, d, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
   FSX_HIP(hipStreamSynchronize(s));
   return 0;
+}
+
+// ns3d.strat: one fused Runge-Kutta stage (stage >= 0) or the drop-in
+// tendency into N (stage = -1, with projection and dealiasing).
+template <typename T>
+int do_strat(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp, void* t1,
+             void* N, const void* src, int has_f, double fcor, double bvf2,
+             int project, int no_vz_kz0, const double* visc, double cfl,
+             double dt_max, bool want_max, hipStream_t s) {
+  const bool use_cfl = cfl > 0 && stage == 0;
+  const bool reduce = use_cfl || want_max;
+  int rc = 0;
+  if (reduce && (rc = cfl_begin(p, s))) return rc;
+  rc = ns3d_front<T>(p, src, has_f, fcor, reduce, s, 1);
+  if (rc) return rc;
+  const Geo g = geo<T>(p, true);
+  {
+    OpYFwd3<T> op;
+    op.in = W2<T>(p);
+    op.out = W1<T>(p);
+    op.ifs = g.fs2;
+    op.ofs = g.fs1;
+    op.ny = p->ny;
+    op.ld1 = g.ld1;
+    op.ld2 = g.ld2;
+    op.ym = g.ym;
+    note(K_YFWD, 6.0 * p->nz * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
+    rc = launch_strided<T, -1>(op, p->ny, p->nz, g.mx, 6, p->tw[1], s);
+    if (rc) return rc;
+  }
+  if (use_cfl && (rc = cfl_end(p, 3, cfl, dt_max, s))) return rc;
+  OpZFwdRKStrat<T> op;
+  op.in = W1<T>(p);
+  op.ifs = g.fs1;
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.ld1 = g.ld1;
+  op.ym = g.ym;
+  op.zm = g.zm;
+  op.mask = p->d_mask;
+  op.dkx = (T)p->dkx;
+  op.dky = (T)p->dky;
+  op.dkz = (T)p->dkz;
+  op.N2 = (T)bvf2;
+  op.rk = make_rk<T>(p, scheme, stage, 4, S, tmp, t1, N, project, no_vz_kz0, visc);
+  const size_t ke = kelems(p);
+  for (int v = 0; v < 4; ++v)
+    op.rk.src[v] = reinterpret_cast<const cpx<T>*>(src) + v * ke;
+  const double kept = (double)p->zmP.nk() * p->ymP.nk() * p->mxP;
+  const double rkb = stage < 0 ? kept * 4 * sizeof(cpx<T>)
+                               : rk_bytes(p, scheme, stage, 4, sizeof(cpx<T>));
+  note(K_ZFWD, 6.0 * g.ym.nk() * g.mx * p->nz * sizeof(cpx<T>) +
+                   2.0 * kept * sizeof(cpx<T>) + rkb);
+  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, 6, p->tw[2], s);
 }
 
 }  // namespace
@@ -811,6 +1012,84 @@ int fsx_rk_stage(fsx_plan p, int scheme, int stage, int nvar, void* S,
                                      project, no_vz_kz0, visc, s)
                       : do_rk<double>(p, scheme, stage, nvar, S, tmp, t1, N,
                                       project, no_vz_kz0, visc, s);
+}
+
+int fsx_ns2d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                      void* t1, double beta, const double* visc, double cfl,
+                      double dt_max, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !S || !t1 || !visc) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 2 || p->nf_max < 4) return fail(FSX_ERR_ARG, "ns2d needs a 2D plan with nf_max >= 4");
+  if (scheme != 4 && scheme != 2) return fail(FSX_ERR_ARG, "scheme must be 2 or 4");
+  if (stage < 0 || stage >= scheme) return fail(FSX_ERR_ARG, "bad stage");
+  if (scheme == 4 && !tmp) return fail(FSX_ERR_ARG, "RK4 needs tmp");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_ns2d_rk<float>(p, scheme, stage, S, tmp, t1, beta, visc,
+                                          cfl, dt_max, s)
+                      : do_ns2d_rk<double>(p, scheme, stage, S, tmp, t1, beta,
+                                           visc, cfl, dt_max, s);
+}
+
+int fsx_ns3d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                      void* t1, int has_f, double f, int project, int no_vz_kz0,
+                      const double* visc, double cfl, double dt_max,
+                      void* stream) {
+  PlanScope scope_(p);
+  if (!p || !S || !t1 || !visc) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3 || p->nf_max < 6) return fail(FSX_ERR_ARG, "ns3d needs a 3D plan with nf_max >= 6");
+  if (scheme != 4 && scheme != 2) return fail(FSX_ERR_ARG, "scheme must be 2 or 4");
+  if (stage < 0 || stage >= scheme) return fail(FSX_ERR_ARG, "bad stage");
+  if (scheme == 4 && !tmp) return fail(FSX_ERR_ARG, "RK4 needs tmp");
+  if (project < 0 || project > 3) return fail(FSX_ERR_ARG, "bad projection kind");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4
+             ? do_ns3d_rk<float>(p, scheme, stage, S, tmp, t1, has_f, f, project,
+                                 no_vz_kz0, visc, cfl, dt_max, s)
+             : do_ns3d_rk<double>(p, scheme, stage, S, tmp, t1, has_f, f, project,
+                                  no_vz_kz0, visc, cfl, dt_max, s);
+}
+
+int fsx_strat_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                       void* t1, int has_f, double f, double N, int project,
+                       int no_vz_kz0, const double* visc, double cfl,
+                       double dt_max, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !S || !t1 || !visc) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3 || p->nf_max < 7)
+    return fail(FSX_ERR_ARG, "ns3d.strat needs a 3D plan with nf_max >= 7");
+  if (scheme != 4 && scheme != 2) return fail(FSX_ERR_ARG, "scheme must be 2 or 4");
+  if (stage < 0 || stage >= scheme) return fail(FSX_ERR_ARG, "bad stage");
+  if (scheme == 4 && !tmp) return fail(FSX_ERR_ARG, "RK4 needs tmp");
+  if (project < 0 || project > 3) return fail(FSX_ERR_ARG, "bad projection kind");
+  hipStream_t s = (hipStream_t)stream;
+  void* src = stage == 0 ? S : t1;
+  return p->prec == 4
+             ? do_strat<float>(p, scheme, stage, S, tmp, t1, nullptr, src, has_f, f,
+                               N * N, project, no_vz_kz0, visc, cfl, dt_max, false, s)
+             : do_strat<double>(p, scheme, stage, S, tmp, t1, nullptr, src, has_f,
+                                f, N * N, project, no_vz_kz0, visc, cfl, dt_max,
+                                false, s);
+}
+
+int fsx_strat_tendency(fsx_plan p, const void* state, void* out_fft, int has_f,
+                       double f, double N, int project, int no_vz_kz0,
+                       int want_maxima, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !state || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3 || p->nf_max < 7)
+    return fail(FSX_ERR_ARG, "ns3d.strat needs a 3D plan with nf_max >= 7");
+  if (project < 0 || project > 3) return fail(FSX_ERR_ARG, "bad projection kind");
+  hipStream_t s = (hipStream_t)stream;
+  const double visc[4] = {0, 0, 0, 0};
+  void* st = const_cast<void*>(state);
+  // stage -1: the fused pass stores the tendency instead of a stage update
+  return p->prec == 4
+             ? do_strat<float>(p, 4, -1, st, nullptr, st, out_fft, state, has_f, f,
+                               N * N, project, no_vz_kz0, visc, 0.0, 0.0,
+                               want_maxima != 0, s)
+             : do_strat<double>(p, 4, -1, st, nullptr, st, out_fft, state, has_f,
+                                f, N * N, project, no_vz_kz0, visc, 0.0, 0.0,
+                                want_maxima != 0, s);
 }
 
 int fsx_project(fsx_plan p, void* a, void* b, void* c, int kind, void* stream) {

@@ -136,6 +136,13 @@ __host__ __device__ __forceinline__ int stage_log_radix(int logN, int logNs) {
   return rem >= 3 ? 3 : rem;
 }
 
+// LDS address with one pad slot per 32 elements: power-of-two strides of
+// the Stockham write patterns then fall on different banks.
+#ifndef FSX_LDS_PAD
+#define FSX_LDS_PAD 1
+#endif
+__device__ __forceinline__ int lpad(int a) { return FSX_LDS_PAD ? a + (a >> 5) : a; }
+
 // Apply the Stockham twiddles W_{Ns R}^{r k} to v[1..R-1].
 // tw is a table exp(-2 pi i m / Ntab) with Ntab = N << twshift.
 template <typename T, int DIR, int R, int LOGR>
@@ -172,7 +179,7 @@ __device__ __forceinline__ void lds_stage_lf(cpx<T>* buf, int logN, int logNs,
       const int k = j & (Ns - 1);
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        v[i][r] = buf[((j + (r << lognb)) << logL) + line];
+        v[i][r] = buf[lpad(((j + (r << lognb)) << logL) + line)];
       twiddle<T, DIR, R, LOGR>(v[i], k, logN, logNs, tw, 0);
       dftR<T, DIR, R>(v[i]);
       ob[i] = ((((j - k) << LOGR) + k) << logL) + line;
@@ -184,13 +191,13 @@ __device__ __forceinline__ void lds_stage_lf(cpx<T>* buf, int logN, int logNs,
     const int b = threadIdx.x + i * kT;
     if (b < total) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[ob[i] + ((r * Ns) << logL)] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lpad(ob[i] + ((r * Ns) << logL))] = v[i][r];
     }
   }
   __syncthreads();
 }
 
-// Full in-LDS FFT over `lines` rows stored at buf[line * ldr + pos]
+// Full in-LDS FFT over `lines` rows stored at buf[line * ldr + lpad(pos)]
 // (row layout, butterfly-fast thread map), all stages.  Used by the row
 // (x) pass.  Caller synchronises before the call.
 template <typename T, int DIR, int EPT, int R, int kT>
@@ -203,7 +210,7 @@ __device__ __forceinline__ void lds_stage_rows(cpx<T>* buf, int logN,
   const int total = lines << lognb;
   const int Ns = 1 << logNs;
   cpx<T> v[NB][R];
-  int ob[NB];
+  int ob[NB], op[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     const int b = threadIdx.x + i * kT;
@@ -213,10 +220,11 @@ __device__ __forceinline__ void lds_stage_rows(cpx<T>* buf, int logN,
       const int k = j & (Ns - 1);
       const cpx<T>* row = buf + line * ldr;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = row[j + (r << lognb)];
+      for (int r = 0; r < R; ++r) v[i][r] = row[lpad(j + (r << lognb))];
       if (logNs > 0) twiddle<T, DIR, R, LOGR>(v[i], k, logN, logNs, tw, twshift);
       dftR<T, DIR, R>(v[i]);
-      ob[i] = line * ldr + ((j - k) << LOGR) + k;
+      ob[i] = line * ldr;
+      op[i] = ((j - k) << LOGR) + k;
     }
   }
   __syncthreads();
@@ -225,7 +233,7 @@ __device__ __forceinline__ void lds_stage_rows(cpx<T>* buf, int logN,
     const int b = threadIdx.x + i * kT;
     if (b < total) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[ob[i] + r * Ns] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[ob[i] + lpad(op[i] + r * Ns)] = v[i][r];
     }
   }
   __syncthreads();

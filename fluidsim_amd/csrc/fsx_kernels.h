@@ -21,6 +21,7 @@
 #pragma once
 
 #include "fsx_fft.h"
+#include "fsx_pointwise.h"
 
 namespace fsx {
 
@@ -86,7 +87,7 @@ __device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
         for (int r = 0; r < 8; ++r) v[r] = src(i * 8 + r, line, j + (r << lognb));
         dft8<T, DIR>(v);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) buf[(((j << 3) + r) << logL) + line] = v[r];
+        for (int r = 0; r < 8; ++r) buf[lpad((((j << 3) + r) << logL) + line)] = v[r];
       }
     }
     __syncthreads();
@@ -111,7 +112,7 @@ __device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
         const int j = b >> logL;
         cpx<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = buf[((j + (r << lognb)) << logL) + line];
+        for (int r = 0; r < R; ++r) v[r] = buf[lpad(((j + (r << lognb)) << logL) + line)];
         twiddle<T, DIR, R, LOGR>(v, j, logN, logN - LOGR, tw, 0);
         dftR<T, DIR, R>(v);
 #pragma unroll
@@ -127,6 +128,29 @@ __device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
   else
     last(std::integral_constant<int, 2>());
   __syncthreads();
+}
+
+// Position of register slot e of tile_fft's last stage: returns false when
+// the slot holds no butterfly output, else its local line and position.
+template <int EPT, int kT>
+__device__ __forceinline__ bool last_slot(int e, int logN, int logL, int& line,
+                                          int& p) {
+  int logR;
+  if (logN <= 3) {
+    logR = logN;
+  } else {
+    int logNs = 3;
+    while (logN - logNs > 3) logNs += 3;
+    logR = logN - logNs;
+  }
+  const int i = e >> logR, r = e & ((1 << logR) - 1);
+  if (i >= (EPT >> logR)) return false;
+  const int b = threadIdx.x + i * kT;
+  const int lognb = logN - logR;
+  if (b >= (1 << (lognb + logL))) return false;
+  line = b & ((1 << logL) - 1);
+  p = (b >> logL) + (r << lognb);
+  return true;
 }
 
 // ---------------------------------------------------------------- strided
@@ -150,7 +174,7 @@ struct PlainRun {
   static constexpr int kEptDiv = 1;
   static constexpr bool kFieldPerBlock = true;
   template <int DIR, int EPT, int kT>
-  __device__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const Self& s = static_cast<const Self&>(*this);
     const int nlines = s.d.nlines;
     const int f = blockIdx.y;
@@ -304,9 +328,10 @@ struct OpZInvCurl {
   T dkx, dky, dkz;
   int has_f;
   T fcor;
+  int with_b;  // ns3d.strat: input 3 (b) transformed into output 6
 
   template <int DIR, int EPT, int kT>
-  __device__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
     const int iy = ym.full(o);
     const T ky = T(ksigned(iy, ny)) * dky;
@@ -369,6 +394,10 @@ struct OpZInvCurl {
           if (has_f && iy == 0 && line0 + l == 0) w.x += fcor;
           st(5, l, p, w);
         });
+    if (with_b)
+      tile_fft<T, DIR, EPT, kT>(
+          buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
+          [&](int, int l, int p, cpx<T> v) { st(6, l, p, v); });
   }
 };
 
@@ -388,7 +417,7 @@ struct OpYInv2NS {
   T dkx, dky;
 
   template <int DIR, int EPT, int kT>
-  __device__ void run(cpx<T>* buf, const cpx<T>* tw, int, int line0) const {
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int, int line0) const {
     const int nlines = d.nlines;
     cpx<T> sr[EPT];
     auto st = [&](int c, int l, int p, cpx<T> v) {
@@ -427,6 +456,162 @@ struct OpYInv2NS {
         buf, tw, d.logN, d.logL,
         [&](int e, int, int p) { return mul_i(scale(sr[e], T(ksigned(p, ny)) * dky)); },
         [&](int, int l, int p, cpx<T> v) { st(3, l, p, v); });
+  }
+};
+
+// ns3d z-pass, forward, with the Runge-Kutta stage folded in: the three
+// tendency components are transformed one after the other (the first two
+// kept in registers), then each kept mode is projected, masked and advanced
+// (rk_point); the tendency never goes to HBM.
+template <typename T>
+struct OpZFwdRK3 {
+  static constexpr int kEptDiv = 2;
+  static constexpr bool kFieldPerBlock = false;
+  StridedDesc<T> d;
+  const cpx<T>* in;  // W1, 3 fields
+  long long ifs;
+  int ny, nkx, ld1;
+  AxisMap ym, zm;
+  const unsigned char* mask;
+  RKArgs<T> rk;
+
+  template <int DIR, int EPT, int kT>
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+    const int nlines = d.nlines;
+    const int nyk = ym.nk();
+    const int iy = ym.full(o);
+    const int iky = ksigned(iy, ny);
+    const double dt = *rk.dt;
+    cpx<T> s0[EPT], s1[EPT], s2[EPT];
+    auto ld = [&](int c, int l, int p) {
+      const int line = line0 + l;
+      if (line >= nlines) return zero<T>();
+      return in[c * ifs + (p * nyk + o) * ld1 + line];
+    };
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
+        [&](int e, int, int, cpx<T> v) { s0[e] = v; });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
+        [&](int e, int, int, cpx<T> v) { s1[e] = v; });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+        [&](int e, int, int, cpx<T> v) { s2[e] = v; });
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      int l, p;
+      if (!last_slot<EPT, kT>(e, d.logN, d.logL, l, p)) continue;
+      const int line = line0 + l;
+      if (line >= nlines || zm.compact(p) < 0) continue;
+      const int idx = (p * ny + iy) * nkx + line;
+      cpx<T> N[3] = {s0[e], s1[e], s2[e]};
+      if (mask && mask[idx]) N[0] = N[1] = N[2] = zero<T>();
+      rk_point<T, 3>(rk, dt, idx, line, iky, ksigned(p, d.N), N);
+    }
+  }
+};
+
+// ns3d.strat z-pass, forward, with the Runge-Kutta stage folded in
+// (fluidsim/solvers/ns3d/strat/solver.py:29-33, 138-216): W1 fields 0-2 are
+// the transforms of v x omega, 3-5 those of v b.  Per kept mode
+//   N_v = (F_x, F_y, F_z + b),  N_b = -i k.(v b)^ - N^2 v_z,
+// with b and v_z of the state the tendency is evaluated on (rk.src); the
+// projection acts on N_v only (rk_point).
+template <typename T>
+struct OpZFwdRKStrat {
+  static constexpr int kEptDiv = 2;
+  static constexpr bool kFieldPerBlock = false;
+  StridedDesc<T> d;
+  const cpx<T>* in;  // W1, 6 fields
+  long long ifs;
+  int ny, nkx, ld1;
+  AxisMap ym, zm;
+  const unsigned char* mask;
+  T dkx, dky, dkz;
+  T N2;
+  RKArgs<T> rk;
+
+  template <int DIR, int EPT, int kT>
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+    const int nlines = d.nlines;
+    const int nyk = ym.nk();
+    const int iy = ym.full(o);
+    const int iky = ksigned(iy, ny);
+    const T ky = T(iky) * dky;
+    const double dt = *rk.dt;
+    cpx<T> s0[EPT], s1[EPT], s2[EPT], sb[EPT];
+    auto ld = [&](int c, int l, int p) {
+      const int line = line0 + l;
+      if (line >= nlines) return zero<T>();
+      return in[c * ifs + (p * nyk + o) * ld1 + line];
+    };
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
+        [&](int e, int, int, cpx<T> v) { s0[e] = v; });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
+        [&](int e, int, int, cpx<T> v) { s1[e] = v; });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+        [&](int e, int, int, cpx<T> v) { s2[e] = v; });
+    // sb = kx G_x + ky G_y + kz G_z  (N_b = -i sb - N^2 v_z)
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
+        [&](int e, int l, int, cpx<T> v) { sb[e] = scale(v, T(line0 + l) * dkx); });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(4, l, p); },
+        [&](int e, int, int, cpx<T> v) { sb[e] = sb[e] + scale(v, ky); });
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(5, l, p); },
+        [&](int e, int, int p, cpx<T> v) {
+          sb[e] = sb[e] + scale(v, T(ksigned(p, d.N)) * dkz);
+        });
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      int l, p;
+      if (!last_slot<EPT, kT>(e, d.logN, d.logL, l, p)) continue;
+      const int line = line0 + l;
+      if (line >= nlines || zm.compact(p) < 0) continue;
+      const int idx = (p * ny + iy) * nkx + line;
+      const cpx<T> bs = rk.src[3][idx], vzs = rk.src[2][idx];
+      cpx<T> N[4] = {s0[e], s1[e], s2[e] + bs,
+                     scale(mul_i(sb[e]), T(-1)) - scale(vzs, N2)};
+      if (mask && mask[idx]) N[0] = N[1] = N[2] = N[3] = zero<T>();
+      rk_point<T, 4>(rk, dt, idx, line, iky, ksigned(p, d.N), N);
+    }
+  }
+};
+
+// ns2d y-pass, forward, with the Runge-Kutta stage folded in.
+template <typename T>
+struct OpYFwd2RK {
+  static constexpr int kEptDiv = 1;
+  static constexpr bool kFieldPerBlock = false;
+  StridedDesc<T> d;
+  const cpx<T>* in;  // W2, 1 field
+  int ny, nkx, ld2;
+  AxisMap ym;
+  const unsigned char* mask;
+  RKArgs<T> rk;
+
+  template <int DIR, int EPT, int kT>
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int, int line0) const {
+    const int nlines = d.nlines;
+    const double dt = *rk.dt;
+    tile_fft<T, DIR, EPT, kT>(
+        buf, tw, d.logN, d.logL,
+        [&](int, int l, int p) {
+          const int line = line0 + l;
+          return line < nlines ? in[p * ld2 + line] : zero<T>();
+        },
+        [&](int, int l, int p, cpx<T> v) {
+          const int line = line0 + l;
+          if (line >= nlines || ym.compact(p) < 0) return;
+          const int idx = p * nkx + line;
+          cpx<T> N[1] = {v};
+          if (mask && mask[idx]) N[0] = zero<T>();
+          rk_point<T, 1>(rk, dt, idx, line, ksigned(p, ny), 0, N);
+        });
   }
 };
 
@@ -476,8 +661,18 @@ __device__ void block_max_atomic(T m0, T m1, T m2, int nv,
   __syncthreads();
 }
 
+// Real view of the padded row layout: real x of line lr.
+__device__ __forceinline__ int rpad(int lr, int x, int ldr) {
+  return 2 * (lr * ldr + lpad(x >> 1)) + (x & 1);
+}
+
+#ifndef FSX_XPASS_WAVES
+#define FSX_XPASS_WAVES 4
+#endif
+// At least FSX_XPASS_WAVES waves per SIMD (caps the VGPR count).
 template <typename T, int EPT, int kT, class Op>
-__global__ __launch_bounds__(kT) void k_xpass(Op op) {
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XPASS_WAVES)))
+void k_xpass(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const XDesc<T>& d = op.d;
   cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -485,6 +680,7 @@ __global__ __launch_bounds__(kT) void k_xpass(Op op) {
   T* bufr = reinterpret_cast<T*>(buf);
   const int M = d.M, logM = d.logM, nx = d.nx;
   const int H = M >> 1, logH = logM - 1;
+  const int ldr = lpad(M);  // padded complex row stride in LDS
   const int row0 = blockIdx.x << d.logRB;
   const int tid = threadIdx.x;
   for (int i = tid; i < nx; i += kT) s_tw[i] = d.tw[i];
@@ -501,7 +697,7 @@ __global__ __launch_bounds__(kT) void k_xpass(Op op) {
       const int lr = idx >> logH;
       const int f = lr >> d.logRB;
       const int row = row0 + (lr & (d.RB - 1));
-      cpx<T>* Z = buf + (lr << logM);
+      cpx<T>* Z = buf + lr * ldr;
       cpx<T> Xk = zero<T>(), Xm = zero<T>(), Xh = zero<T>();
       if (row < d.nrows) {
         const cpx<T>* src = op.in + f * d.ifs + (long long)row * d.ld_in;
@@ -515,17 +711,17 @@ __global__ __launch_bounds__(kT) void k_xpass(Op op) {
       }
       const cpx<T> w = s_tw[k];
       const cpx<T> dk = Xk - conj(Xm), dm = Xm - conj(Xk);
-      Z[k] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
+      Z[lpad(k)] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
       if (k == 0) {
         const cpx<T> dh = Xh - conj(Xh);
-        Z[H] = (Xh + conj(Xh)) + mul_i(conj(s_tw[H]) * dh);
+        Z[lpad(H)] = (Xh + conj(Xh)) + mul_i(conj(s_tw[H]) * dh);
       } else {
         // conj(W^(M-k)) = -W^k
-        Z[M - k] = (Xm + conj(Xk)) - mul_i(w * dm);
+        Z[lpad(M - k)] = (Xm + conj(Xk)) - mul_i(w * dm);
       }
     }
     __syncthreads();
-    lds_fft_rows<T, +1, EPT, kT>(buf, logM, d.nf_in << d.logRB, M, s_tw, 1);
+    lds_fft_rows<T, +1, EPT, kT>(buf, logM, d.nf_in << d.logRB, ldr, s_tw, 1);
   } else {
     const int n_in = (d.nf_in << d.logRB) << d.lognx;
     for (int idx = tid; idx < n_in; idx += kT) {
@@ -533,7 +729,8 @@ __global__ __launch_bounds__(kT) void k_xpass(Op op) {
       const int lr = idx >> d.lognx;
       const int f = lr >> d.logRB;
       const int row = row0 + (lr & (d.RB - 1));
-      bufr[idx] = row < d.nrows ? op.inr[f * d.ifs + (long long)row * nx + x] : T(0);
+      bufr[rpad(lr, x, ldr)] =
+          row < d.nrows ? op.inr[f * d.ifs + (long long)row * nx + x] : T(0);
     }
     __syncthreads();
   }
@@ -545,13 +742,13 @@ __global__ __launch_bounds__(kT) void k_xpass(Op op) {
       const int lr = idx >> d.lognx;
       const int f = lr >> d.logRB;
       const int row = row0 + (lr & (d.RB - 1));
-      if (row < d.nrows) op.outr[f * d.ofs + (long long)row * nx + x] = bufr[idx];
+      if (row < d.nrows) op.outr[f * d.ofs + (long long)row * nx + x] = bufr[rpad(lr, x, ldr)];
     }
     return;
   } else {
-    op.physical(bufr, row0);
+    op.physical(bufr, row0, ldr);
     __syncthreads();
-    lds_fft_rows<T, -1, EPT, kT>(buf, logM, d.nf_out << d.logRB, M, s_tw, 1);
+    lds_fft_rows<T, -1, EPT, kT>(buf, logM, d.nf_out << d.logRB, ldr, s_tw, 1);
     // r2c post-processing, one thread per pair (k, M - k):
     // E = (Z[k] + conj Z[M-k]) / 2, O = -i (Z[k] - conj Z[M-k]) / 2,
     // X[k] = E + W^k O, X[M-k] = conj(E - W^k O); k = 0 also does M/2.
@@ -562,18 +759,18 @@ __global__ __launch_bounds__(kT) void k_xpass(Op op) {
       const int f = lr >> d.logRB;
       const int row = row0 + (lr & (d.RB - 1));
       if (row >= d.nrows) continue;
-      const cpx<T>* Z = buf + (lr << logM);
+      const cpx<T>* Z = buf + lr * ldr;
       cpx<T>* dst = op.out + f * d.ofs + (long long)row * d.ld_out;
       const T h = T(0.5);
       {
-        const cpx<T> Zk = Z[k], Zm = conj(Z[(M - k) & (M - 1)]);
+        const cpx<T> Zk = Z[lpad(k)], Zm = conj(Z[lpad((M - k) & (M - 1))]);
         const cpx<T> E = scale(Zk + Zm, h), dd = Zk - Zm;
         const cpx<T> P = s_tw[k] * mk<T>(dd.y * h, -dd.x * h);
         if (k < d.mx_out) dst[k] = scale(E + P, d.scale);
         if (M - k < d.mx_out) dst[M - k] = scale(conj(E - P), d.scale);
       }
       if (k == 0 && H < d.mx_out) {
-        const cpx<T> Zk = Z[H], Zm = conj(Z[H]);
+        const cpx<T> Zk = Z[lpad(H)], Zm = conj(Z[lpad(H)]);
         const cpx<T> E = scale(Zk + Zm, h), dd = Zk - Zm;
         const cpx<T> P = s_tw[H] * mk<T>(dd.y * h, -dd.x * h);
         dst[H] = scale(E + P, d.scale);
@@ -591,7 +788,7 @@ struct OpXC2R {
   const T* inr;
   T* outr;
   cpx<T>* out;
-  __device__ void physical(T*, int) const {}
+  __device__ void physical(T*, int, int) const {}
 };
 
 // plain r2c rows: X rows -> W2 rows, scaled by 1/(number of points)
@@ -603,7 +800,7 @@ struct OpXR2C {
   const T* inr;
   T* outr;
   cpx<T>* out;
-  __device__ void physical(T*, int) const {}
+  __device__ void physical(T*, int, int) const {}
 };
 
 // ns3d rows: (v, omega) -> v x omega, optional CFL maxima of |v_i|.
@@ -616,13 +813,12 @@ struct OpXNS3D {
   T* outr;
   cpx<T>* out;
   unsigned long long* maxima;  // 3 slots or null
-  __device__ void physical(T* b, int row0) const {
-    const int nx = d.nx;
+  __device__ void physical(T* b, int row0, int ldr) const {
     const int n = d.RB << d.lognx;
-    const int s = n;  // reals per field
+    const int s = 2 * ldr * d.RB;  // reals per field
     T m0 = 0, m1 = 0, m2 = 0;
     for (int idx = threadIdx.x; idx < n; idx += kT) {
-      T* p = b + idx;
+      T* p = b + rpad(idx >> d.lognx, idx & (d.nx - 1), ldr);
       const T vx = p[0], vy = p[s], vz = p[2 * s];
       const T wx = p[3 * s], wy = p[4 * s], wz = p[5 * s];
       p[0] = vy * wz - vz * wy;
@@ -634,7 +830,40 @@ struct OpXNS3D {
         m2 = fmax(m2, fabs(vz));
       }
     }
-    (void)nx;
+    if (maxima) block_max_atomic<T, kT>(m0, m1, m2, 3, maxima);
+  }
+};
+
+// ns3d.strat rows: (v, omega, b) -> (v x omega, v b), CFL maxima of |v_i|.
+template <typename T, int kT>
+struct OpXStrat {
+  static constexpr bool kInv = true, kStoreReal = false;
+  XDesc<T> d;
+  const cpx<T>* in;
+  const T* inr;
+  T* outr;
+  cpx<T>* out;
+  unsigned long long* maxima;  // 3 slots or null
+  __device__ void physical(T* b, int row0, int ldr) const {
+    const int n = d.RB << d.lognx;
+    const int s = 2 * ldr * d.RB;
+    T m0 = 0, m1 = 0, m2 = 0;
+    for (int idx = threadIdx.x; idx < n; idx += kT) {
+      T* p = b + rpad(idx >> d.lognx, idx & (d.nx - 1), ldr);
+      const T vx = p[0], vy = p[s], vz = p[2 * s];
+      const T wx = p[3 * s], wy = p[4 * s], wz = p[5 * s], bb = p[6 * s];
+      p[0] = vy * wz - vz * wy;
+      p[s] = vz * wx - vx * wz;
+      p[2 * s] = vx * wy - vy * wx;
+      p[3 * s] = vx * bb;
+      p[4 * s] = vy * bb;
+      p[5 * s] = vz * bb;
+      if (row0 + (idx >> d.lognx) < d.nrows) {
+        m0 = fmax(m0, fabs(vx));
+        m1 = fmax(m1, fabs(vy));
+        m2 = fmax(m2, fabs(vz));
+      }
+    }
     if (maxima) block_max_atomic<T, kT>(m0, m1, m2, 3, maxima);
   }
 };
@@ -650,12 +879,12 @@ struct OpXNS2D {
   cpx<T>* out;
   unsigned long long* maxima;  // 2 slots or null
   T beta;
-  __device__ void physical(T* b, int row0) const {
+  __device__ void physical(T* b, int row0, int ldr) const {
     const int n = d.RB << d.lognx;
-    const int s = n;
+    const int s = 2 * ldr * d.RB;
     T m0 = 0, m1 = 0;
     for (int idx = threadIdx.x; idx < n; idx += kT) {
-      T* p = b + idx;
+      T* p = b + rpad(idx >> d.lognx, idx & (d.nx - 1), ldr);
       const T ux = p[0], uy = p[s], px = p[2 * s], py = p[3 * s];
       p[0] = beta == T(0) ? -ux * px - uy * py : -ux * px - uy * (py + beta);
       if (row0 + (idx >> d.lognx) < d.nrows) {

@@ -99,9 +99,10 @@ struct RKArgs {
   cpx<T>* tmp[4];
   cpx<T>* t1[4];
   cpx<T>* Nt[4];  // tendencies (projected in place when project != 0)
+  const cpx<T>* src[4];  // state the tendency is evaluated on (fused strat)
   int nvar;
   int scheme;  // 4: RK4, 2: RK2
-  int stage;
+  int stage;    // < 0: store the (projected) tendency into Nt, no update
   int project;  // projection kind applied to variables 0..2 (3D)
   int no_vz_kz0;
   const double* dt;
@@ -110,65 +111,105 @@ struct RKArgs {
   unsigned int* nanflag;
 };
 
-// One integrating-factor Runge-Kutta stage over the kept modes
-// (fluidsim/base/time_stepping/pseudo_spect.py:51-68, 466-517, 898-984).
-template <typename T>
-__global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
-  const size_t n = a.box.count();
-  const double dt = *a.dt;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    int ikx, iky, ikz;
-    const size_t idx = a.box.locate(i, ikx, iky, ikz);
-    const double sig = freq_diss(a.box, a.visc, ikx, iky, ikz);
-    const T e = T(exp(-dt * sig));
-    const T e2 = T(exp(-dt / 2 * sig));
-    cpx<T> N[4];
-    for (int v = 0; v < a.nvar; ++v) N[v] = a.Nt[v][idx];
+// Integrating-factor Runge-Kutta update of one mode
+// (fluidsim/base/time_stepping/pseudo_spect.py:51-68, 466-517, 898-984):
+// N holds the NV tendencies of the mode (projected here when a.project),
+// E = exp(-dt sigma), E2 = exp(-dt/2 sigma) in FP64.
+template <typename T, int NV>
+__device__ __forceinline__ void rk_point(const RKArgs<T>& a, double dt,
+                                         int idx, int ikx, int iky, int ikz,
+                                         cpx<T>* N) {
+  const Box& bx = a.box;
+  const double sig = freq_diss(bx, a.visc, ikx, iky, ikz);
+  const T e = T(exp(-dt * sig));
+  const T e2 = T(exp(-dt / 2 * sig));
+  const T c6 = T(dt / 6), c3 = T(dt / 3), c2 = T(dt / 2), c1 = T(dt);
+  if constexpr (NV >= 3) {
     if (a.project != PROJ_NONE) {
-      project3(N[0], N[1], N[2], a.project, ikx, iky, ikz, a.box);
+      project3(N[0], N[1], N[2], a.project, ikx, iky, ikz, bx);
       if (a.no_vz_kz0 && ikz == 0) {
         N[2] = zero<T>();
-        if (a.nvar > 3) N[3] = zero<T>();
+        if (NV > 3) N[3] = zero<T>();
       }
     }
-    cpx<T> Snew[4];
-    for (int v = 0; v < a.nvar; ++v) {
-      const cpx<T> S = a.S[v][idx];
-      const cpx<T> Nv = N[v];
-      if (a.scheme == 4) {
-        if (a.stage == 0) {
-          a.tmp[v][idx] = scale(S + scale(Nv, T(dt / 6)), e);
-          a.t1[v][idx] = scale(S + scale(Nv, T(dt / 2)), e2);
-        } else if (a.stage == 1) {
-          a.tmp[v][idx] = a.tmp[v][idx] + scale(Nv, T(dt / 3) * e2);
-          a.t1[v][idx] = scale(S, e2) + scale(Nv, T(dt / 2));
-        } else if (a.stage == 2) {
-          a.tmp[v][idx] = a.tmp[v][idx] + scale(Nv, T(dt / 3) * e2);
-          a.t1[v][idx] = scale(S, e) + scale(Nv, T(dt) * e2);
-        } else {
-          Snew[v] = a.tmp[v][idx] + scale(Nv, T(dt / 6));
-        }
+  }
+  if (a.stage < 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a.Nt[v][idx] = N[v];
+    return;
+  }
+  cpx<T> Sn[NV];
+  bool last = false;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const cpx<T> Nv = N[v];
+    if (a.scheme == 4) {
+      if (a.stage == 0) {
+        const cpx<T> S = a.S[v][idx];
+        a.tmp[v][idx] = scale(S + scale(Nv, c6), e);
+        a.t1[v][idx] = scale(S + scale(Nv, c2), e2);
+      } else if (a.stage == 1) {
+        const cpx<T> S = a.S[v][idx];
+        a.tmp[v][idx] = a.tmp[v][idx] + scale(Nv, c3 * e2);
+        a.t1[v][idx] = scale(S, e2) + scale(Nv, c2);
+      } else if (a.stage == 2) {
+        const cpx<T> S = a.S[v][idx];
+        a.tmp[v][idx] = a.tmp[v][idx] + scale(Nv, c3 * e2);
+        a.t1[v][idx] = scale(S, e) + scale(Nv, c1 * e2);
       } else {
-        if (a.stage == 0) {
-          a.t1[v][idx] = scale(S + scale(Nv, T(dt / 2)), e2);
-        } else {
-          Snew[v] = scale(S, e) + scale(Nv, T(dt) * e2);
+        Sn[v] = a.tmp[v][idx] + scale(Nv, c6);
+        last = true;
+      }
+    } else {
+      const cpx<T> S = a.S[v][idx];
+      if (a.stage == 0) {
+        a.t1[v][idx] = scale(S + scale(Nv, c2), e2);
+      } else {
+        Sn[v] = scale(S, e) + scale(Nv, c1 * e2);
+        last = true;
+      }
+    }
+  }
+  if (last) {
+    if constexpr (NV >= 3) {
+      if (a.project != PROJ_NONE) {
+        project3(Sn[0], Sn[1], Sn[2], a.project, ikx, iky, ikz, bx);
+        if (a.no_vz_kz0 && ikz == 0) {
+          Sn[2] = zero<T>();
+          if (NV > 3) Sn[3] = zero<T>();
         }
       }
     }
-    const bool last = (a.scheme == 4 && a.stage == 3) ||
-                      (a.scheme == 2 && a.stage == 1);
-    if (last) {
-      if (a.project != PROJ_NONE) {
-        project3(Snew[0], Snew[1], Snew[2], a.project, ikx, iky, ikz, a.box);
-        if (a.no_vz_kz0 && ikz == 0) {
-          Snew[2] = zero<T>();
-          if (a.nvar > 3) Snew[3] = zero<T>();
-        }
-      }
-      for (int v = 0; v < a.nvar; ++v) a.S[v][idx] = Snew[v];
-      if (isnan(Snew[0].x) || isnan(Snew[0].y)) atomicOr(a.nanflag, 1u);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a.S[v][idx] = Sn[v];
+    if (isnan(Sn[0].x) || isnan(Sn[0].y)) atomicOr(a.nanflag, 1u);
+  }
+}
+
+// One Runge-Kutta stage over the kept modes from a tendency array.
+// One wavefront per kept (kz, ky) row, lanes along kx.
+template <typename T, int NV>
+__global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
+  const Box& bx = a.box;
+  const int nyk = bx.ym.nk();
+  const int nrow = (bx.dim == 3 ? bx.zm.nk() : 1) * nyk;
+  const int lane = threadIdx.x & 63;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const double dt = *a.dt;
+  for (int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < nrow;
+       row += nwaves) {
+    const int jz = row / nyk, jy = row - jz * nyk;
+    const int iy = bx.ym.full(jy);
+    const int iz = bx.dim == 3 ? bx.zm.full(jz) : 0;
+    const int iky = ksigned(iy, bx.ny);
+    const int ikz = bx.dim == 3 ? ksigned(iz, bx.nz) : 0;
+    const int base = (iz * bx.ny + iy) * bx.nkx;
+    for (int ikx = lane; ikx < bx.mx; ikx += 64) {
+      const int idx = base + ikx;
+      cpx<T> N[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) N[v] = a.Nt[v][idx];
+      rk_point<T, NV>(a, dt, idx, ikx, iky, ikz, N);
     }
   }
 }

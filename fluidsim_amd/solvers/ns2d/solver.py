@@ -5,6 +5,7 @@ Drop-in for fluidsim.solvers.ns2d.solver (fluidsim/solvers/ns2d/solver.py):
 d_t rot_hat = N(rot) - sigma(k) rot_hat with N = -u.grad(rot) - beta v.
 """
 
+from ... import _lib
 from ...base.solvers import SimulBasePseudoSpectral, InfoSolverPseudoSpectral
 from ...setofvariables import SetOfVariables
 
@@ -36,6 +37,12 @@ class Simul(SimulBasePseudoSpectral):
     def _engine_tendency(self, rot_fft, out_fft, want_maxima=False):
         """Fused device tendency of one vorticity field (tensors (1, *K))."""
         self.oper.ns2d_tendency(rot_fft, out_fft, self.params.beta, want_maxima)
+
+    def _engine_rk_stage(self, scheme, stage, S, tmp, t1, visc, cfl, dt_max):
+        """One fused Runge-Kutta stage on the device (fsx_ns2d_rk_stage)."""
+        _lib.call("fsx_ns2d_rk_stage", self.oper.plan.handle, scheme, stage,
+                  _lib.dptr(S), _lib.dptr(tmp), _lib.dptr(t1),
+                  float(self.params.beta), visc, cfl, dt_max, _lib.stream_ptr())
 
     def tendencies_nonlin(self, state_spect=None, old=None):
         """fluidsim/solvers/ns2d/solver.py:111-194.

@@ -6,6 +6,7 @@ N(v) = P(fft(v x omega)) with omega = curl v (+ f e_z at k = 0), P the Leray
 projection (or toroidal / poloidal with params.projection).
 """
 
+from ... import _lib
 from ...base.solvers import SimulBasePseudoSpectral, InfoSolverPseudoSpectral3D
 from ...operators.operators3d import PROJ
 from ...setofvariables import SetOfVariables
@@ -57,6 +58,14 @@ class Simul(SimulBasePseudoSpectral):
         folded into the Runge-Kutta stage)."""
         self.oper.ns3d_tendency(v_fft, out_fft, f=self.params.f,
                                 want_maxima=want_maxima)
+
+    def _engine_rk_stage(self, scheme, stage, S, tmp, t1, visc, cfl, dt_max):
+        """One fused Runge-Kutta stage on the device (fsx_ns3d_rk_stage)."""
+        f = self.params.f
+        _lib.call("fsx_ns3d_rk_stage", self.oper.plan.handle, scheme, stage,
+                  _lib.dptr(S), _lib.dptr(tmp), _lib.dptr(t1), int(f is not None),
+                  float(f) if f is not None else 0.0, self._engine_projection,
+                  int(self.no_vz_kz0), visc, cfl, dt_max, _lib.stream_ptr())
 
     def project_state_spect(self, state_spect):
         """solver.py:255-263."""

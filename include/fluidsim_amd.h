@@ -103,6 +103,46 @@ int fsx_rk_stage(fsx_plan p, int scheme, int stage, int nvar, void* S,
                  void* tmp, void* t1, void* N, int project, int no_vz_kz0,
                  const double* visc, void* stream);
 
+/* Fused Runge-Kutta stage of ns2d: N(src) with src = S at stage 0 and t1
+ * otherwise, then the stage algebra of fsx_rk_stage applied in the last
+ * forward pass, so the tendency never goes to HBM.  cfl > 0 at stage 0: the
+ * maxima of |u|, |v| of this tendency update the device dt first
+ * (fsx_cfl_update rule, deltat_max = dt_max).
+ * Replaces one stage of _time_step_RK4 / _time_step_RK2 with
+ * Simul.tendencies_nonlin (fluidsim/base/time_stepping/pseudo_spect.py
+ * :466-517, 798-984; fluidsim/solvers/ns2d/solver.py:111-194) and the CFL of
+ * fluidsim/base/time_stepping/base.py:320-366. */
+int fsx_ns2d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                      void* t1, double beta, const double* visc, double cfl,
+                      double dt_max, void* stream);
+
+/* Fused Runge-Kutta stage of ns3d (3 velocity fields), same contract as
+ * fsx_ns2d_rk_stage with the tendency of fluidsim/solvers/ns3d/solver.py
+ * :180-253 and the projection of fsx_rk_stage. */
+int fsx_ns3d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                      void* t1, int has_f, double f, int project, int no_vz_kz0,
+                      const double* visc, double cfl, double dt_max,
+                      void* stream);
+
+/* Fused Runge-Kutta stage of ns3d.strat (4 fields: vx, vy, vz, b) with
+ * Brunt-Vaisala frequency N: tendency
+ *   N_v = P(fft(v x omega) + b e_z),  N_b = -i k.fft(v b) - N^2 v_z
+ * (fluidsim/solvers/ns3d/strat/solver.py:29-33, 138-216), same contract as
+ * fsx_ns3d_rk_stage. */
+int fsx_strat_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                       void* t1, int has_f, double f, double N, int project,
+                       int no_vz_kz0, const double* visc, double cfl,
+                       double dt_max, void* stream);
+
+/* ns3d.strat tendency of `state` (4 fields) into out_fft (4 fields),
+ * projected (kind `project`) and dealiased; only kept modes are written.
+ * Replaces Simul.tendencies_nonlin of
+ * fluidsim/solvers/ns3d/strat/solver.py:138-216.  want_maxima stores
+ * max|v_i| for fsx_cfl_update. */
+int fsx_strat_tendency(fsx_plan p, const void* state, void* out_fft, int has_f,
+                       double f, double N, int project, int no_vz_kz0,
+                       int want_maxima, void* stream);
+
 /* In-place projection of (a, b, c) over all modes: kind 1 project_perpk3d,
  * 2 project_toroidal, 3 project_poloidal
  * (fluidsim/operators/operators3d.py:787-937, fluidfft project_perpk3d). */

@@ -203,3 +203,22 @@ def test_ns3d_tendency_energy_conservation_api():
         T = T + (tend.get_var(key).conj() * sim.state.get_var(key)).real
     ratio = sim.oper.sum_wavenumbers(T) / sim.oper.sum_wavenumbers(T.abs())
     assert abs(ratio) < 1e-14
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fused_and_separate_stages_agree(dim):
+    """The fused stage path (tendency + RK algebra in one pass sequence) and
+    the separate tendency / stage kernels give the same state (FP64, CFL)."""
+    out = []
+    for fused in (True, False):
+        if dim == 2:
+            Simul, params = ns2d_params(64, it_end=6)
+            sim = Simul(params)
+            sim.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(64, 64, seed=3, kmax=12))
+        else:
+            sim, _ = ns3d_sim(32, it_end=4, f=0.7)
+        sim.time_stepping.fused = fused
+        sim.time_stepping.start()
+        out.append((sim.state.state_spect.to_numpy(), sim.time_stepping.t))
+    assert rel_l2(out[0][0], out[1][0]) < 1e-13
+    assert out[0][1] == out[1][1]

@@ -1,0 +1,92 @@
+"""GPU parity of the ns3d.strat engine path against the FP64 oracle
+(fluidsim/solvers/ns3d/strat/solver.py:29-33, 138-216).
+
+Bars: FP64 tendency 1e-12 relative L2; FP64 state after CFL steps 1e-11;
+FP32 state error vs FP64 oracle < 1e-5."""
+
+import numpy as np
+import pytest
+
+from oracle.operators import Oper3D
+from oracle.solvers import NS3DStrat
+from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft
+
+from .gpu_helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def strat_sim(n, precision="float64", N=4.0, nu_2=1e-2, it_end=5, use_cfl=True,
+              seed=0, **extra):
+    from fluidsim_amd.solvers.ns3d.strat.solver import Simul
+
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = params.oper.nz = n
+    params.oper.precision = precision
+    params.N = N
+    params.nu_2 = nu_2
+    for k, v in extra.items():
+        params[k] = v
+    params.time_stepping.USE_CFL = use_cfl
+    params.time_stepping.cfl_coef = 0.5
+    params.time_stepping.deltat0 = 0.01
+    params.time_stepping.USE_T_END = False
+    params.time_stepping.it_end = it_end
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    sim = Simul(params)
+    vv = ns3d_spectrum_vv_fft(n, seed=seed)
+    b = ns3d_spectrum_vv_fft(n, seed=seed + 100)[0] * 0.5
+    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2], b_fft=b)
+    return sim, np.array(list(vv) + [b])
+
+
+def test_strat_tendency_vs_oracle_and_energy():
+    n, N = 16, 4.0
+    sim, S0 = strat_sim(n, N=N)
+    tend = sim.tendencies_nonlin(state_spect=sim.state.state_spect).to_numpy()
+    orc = NS3DStrat(Oper3D(n, n, n), N=N)
+    ref = orc.tendencies_nonlin(S0.copy())
+    err = rel_l2(tend, ref)
+    print(f"strat tendency rel L2 = {err:.3e}")
+    assert err < 1e-12
+    # energy (E_K + E_A) conservation by the tendency
+    oper = orc.oper
+    T = sum(np.real(tend[i].conj() * S0[i]) for i in range(3))
+    T = T + np.real(tend[3].conj() * S0[3]) / N**2
+    assert abs(oper.sum_wavenumbers(T) / oper.sum_wavenumbers(np.abs(T))) < 1e-13
+    EK, EA = sim.output.compute_energies()
+    oEK, oEA = NS3DStrat(oper, N=N), None
+    oEK.state_spect[:] = S0
+    rEK, rEA = oEK.compute_energies()
+    assert EK == pytest.approx(rEK, rel=1e-12) and EA == pytest.approx(rEA, rel=1e-12)
+
+
+@pytest.mark.parametrize("extra", [{}, {"f": 1.0}])
+def test_strat_rk4_cfl_steps_vs_oracle(extra):
+    n, nsteps, N = 32, 6, 4.0
+    sim, S0 = strat_sim(n, N=N, it_end=nsteps, **extra)
+    sim.time_stepping.start()
+    got = sim.state.state_spect.to_numpy()
+    orc = NS3DStrat(Oper3D(n, n, n), N=N, nu_2=1e-2, USE_CFL=True, cfl_coef=0.5,
+                    f=extra.get("f"))
+    orc.state_spect[:] = S0
+    orc.run(nsteps)
+    err = rel_l2(got, orc.state_spect)
+    print(f"strat FP64 {n}^3 {nsteps} steps {extra}: rel L2 = {err:.3e}")
+    assert err <= 1e-11
+    assert sim.time_stepping.t == pytest.approx(orc.t, rel=1e-13)
+
+
+def test_strat_fp32_vs_fp64_oracle():
+    n, nsteps, N = 32, 8, 4.0
+    sim, S0 = strat_sim(n, precision="float32", N=N, it_end=nsteps, use_cfl=False)
+    sim.time_stepping.start()
+    got = sim.state.state_spect.to_numpy()
+    orc = NS3DStrat(Oper3D(n, n, n), N=N, nu_2=1e-2, deltat0=0.01)
+    orc.state_spect[:] = S0
+    orc.run(nsteps)
+    err = rel_l2(got, orc.state_spect)
+    print(f"strat FP32 {n}^3 {nsteps} steps: rel L2 vs FP64 oracle = {err:.3e}")
+    assert err < 1e-5
