@@ -17,6 +17,7 @@
 #include "fsx_fft.h"
 #include "fsx_kernels.h"
 #include "fsx_pointwise.h"
+#include "fsx_rows.h"
 
 using namespace fsx;
 
@@ -192,7 +193,7 @@ int set_lds(K kern, size_t lds) {
 
 template <typename T, int DIR, class Op>
 int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw,
-                   hipStream_t s) {
+                   hipStream_t s, bool kio = false) {
   constexpr int E = ept<T>() / Op::kEptDiv >= 8 ? ept<T>() / Op::kEptDiv : 8;
   const int tile = E * kThr;
   if (N > tile) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
@@ -211,6 +212,7 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   op.d.nlines = nlines;
   op.d.nf = nf;
   op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
+  op.d.kio = kio;
   if (nouter <= 0 || nlines <= 0 || nf <= 0) return 0;
   const long long nchunks = (nlines + L - 1) / L;
   const long long blocks = nchunks * nouter;
@@ -277,6 +279,96 @@ int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kXThr), lds, s, op);
   FSX_HIP(hipGetLastError());
   return tm_post(s);
+}
+
+// Register-resident fused row pass (k_xrow): 2^LOGP points per lane
+// (8 in FP32, 4 in FP64), a row within one wave.  Returns 1 when the row
+// length is outside what it covers (the caller then uses k_xpass).
+#ifndef FSX_XROW
+#define FSX_XROW 1
+#endif
+#ifndef FSX_XROW_THREADS
+#define FSX_XROW_THREADS 256
+#endif
+#ifndef FSX_XROW_BLOCKS_PER_CU
+#define FSX_XROW_BLOCKS_PER_CU 4
+#endif
+constexpr int kXRThr = FSX_XROW_THREADS;
+
+#ifndef FSX_XR_LOGP32
+#define FSX_XR_LOGP32 3
+#endif
+#ifndef FSX_XR_LOGP64
+#define FSX_XR_LOGP64 2
+#endif
+template <typename T>
+constexpr int xrow_logp() {
+  return sizeof(T) == 4 ? FSX_XR_LOGP32 : FSX_XR_LOGP64;
+}
+
+template <typename T, int LOGM, class Op>
+int launch_xrow_m(const XRDesc<T>& d, const Op& op, hipStream_t s) {
+  constexpr int LOGP = xrow_logp<T>();
+  using PL = RowPlan<LOGM, LOGP>;
+  constexpr int RPB = (kXRThr / 64) * (64 / PL::G);
+  const size_t lds = ((size_t)RPB * xrow_ld<T>(PL::M) + (FSX_XR_TWLDS ? 2 * PL::M : 0)) *
+                     sizeof(cpx<T>);
+  auto kern = k_xrow<T, LOGM, LOGP, kXRThr, Op>;
+  static size_t lds_set = 0;
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "row pass needs more LDS than a workgroup has");
+  if (lds > lds_set) {
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
+    lds_set = lds;
+  }
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    FSX_HIP(hipGetDevice(&dev));
+    FSX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ncu <= 0) ncu = 256;
+  }
+  const long long need = (d.nrows + RPB - 1) / RPB;
+  const long long cap = (long long)ncu * FSX_XROW_BLOCKS_PER_CU;
+  const unsigned blocks = (unsigned)(need < cap ? need : cap);
+  if (d.nrows <= 0) return 0;
+  int rc = tm_pre(s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kXRThr), lds, s, d, op);
+  FSX_HIP(hipGetLastError());
+  return tm_post(s);
+}
+
+template <typename T, class Op>
+int launch_xrow(int nx, int nrows, int mx_in, int ld_in, int mx_out, int ld_out,
+                long long ifs, long long ofs, T scl, const void* in, void* out,
+                const void* tw, unsigned long long* maxima, const Op& op,
+                hipStream_t s) {
+  if (!FSX_XROW) return 1;
+  XRDesc<T> d;
+  d.nrows = nrows;
+  d.mx_in = mx_in;
+  d.ld_in = ld_in;
+  d.mx_out = mx_out;
+  d.ld_out = ld_out;
+  d.ifs = ifs;
+  d.ofs = ofs;
+  d.scale = scl;
+  d.in = reinterpret_cast<const cpx<T>*>(in);
+  d.out = reinterpret_cast<cpx<T>*>(out);
+  d.tw = reinterpret_cast<const cpx<T>*>(tw);
+  d.maxima = maxima;
+  constexpr int LOGP = xrow_logp<T>();
+  switch (ilog2(nx / 2) - LOGP) {
+    case 0: return launch_xrow_m<T, LOGP + 0>(d, op, s);
+    case 1: return launch_xrow_m<T, LOGP + 1>(d, op, s);
+    case 2: return launch_xrow_m<T, LOGP + 2>(d, op, s);
+    case 3: return launch_xrow_m<T, LOGP + 3>(d, op, s);
+    case 4: return launch_xrow_m<T, LOGP + 4>(d, op, s);
+    case 5: return launch_xrow_m<T, LOGP + 5>(d, op, s);
+    case 6: return launch_xrow_m<T, LOGP + 6>(d, op, s);
+    default: return 1;
+  }
 }
 
 Box make_box(const fsx_plan_s* p, bool pruned) {
@@ -352,7 +444,7 @@ int inverse_to_w2(fsx_plan_s* p, const cpx<T>* kin, long long kfs, int nf,
     op.ld2 = g.ld2;
     op.ym = g.ym;
     note(K_YINV, (double)nf * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
-    return launch_strided<T, +1>(op, p->ny, 1, g.mx, nf, p->tw[1], s);
+    return launch_strided<T, +1>(op, p->ny, 1, g.mx, nf, p->tw[1], s, true);
   }
   {
     OpZInv<T> op;
@@ -366,7 +458,7 @@ int inverse_to_w2(fsx_plan_s* p, const cpx<T>* kin, long long kfs, int nf,
     op.ym = g.ym;
     op.zm = g.zm;
     note(K_ZINV, (double)nf * g.ym.nk() * g.mx * (g.zm.nk() + p->nz) * sizeof(cpx<T>));
-    int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, nf, p->tw[2], s);
+    int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, nf, p->tw[2], s, true);
     if (rc) return rc;
   }
   OpYInv3<T> op;
@@ -399,7 +491,7 @@ int forward_from_w2(fsx_plan_s* p, cpx<T>* kout, long long kfs, int nf,
     op.ym = g.ym;
     op.mask = mask;
     note(K_YFWD, (double)nf * g.mx * (g.ym.nk() + p->ny) * sizeof(cpx<T>));
-    return launch_strided<T, -1>(op, p->ny, 1, g.mx, nf, p->tw[1], s);
+    return launch_strided<T, -1>(op, p->ny, 1, g.mx, nf, p->tw[1], s, true);
   }
   {
     OpYFwd3<T> op;
@@ -427,7 +519,7 @@ int forward_from_w2(fsx_plan_s* p, cpx<T>* kout, long long kfs, int nf,
   op.zm = g.zm;
   op.mask = mask;
   note(K_ZFWD, (double)nf * g.ym.nk() * g.mx * (p->nz + g.zm.nk()) * sizeof(cpx<T>));
-  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, nf, p->tw[2], s);
+  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, nf, p->tw[2], s, true);
 }
 
 template <typename T>
@@ -488,16 +580,22 @@ int ns2d_front(fsx_plan_s* p, const void* rot, double beta, int want_maxima,
   op.dkx = (T)p->dkx;
   op.dky = (T)p->dky;
   note(K_YINV, (double)g.mx * (g.ym.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
-  int rc = launch_strided<T, +1>(op, p->ny, 1, g.mx, 4, p->tw[1], s);
+  int rc = launch_strided<T, +1>(op, p->ny, 1, g.mx, 4, p->tw[1], s, true);
   if (rc) return rc;
+  unsigned long long* mxs =
+      want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
+  const double npts = (double)p->nx * p->ny;
+  note(K_XPASS, 5.0 * p->ny * g.mx * sizeof(cpx<T>));
+  XRNS2D<T> xr;
+  xr.beta = (T)beta;
+  rc = launch_xrow<T>(p->nx, p->ny, g.mx, g.ld2, g.mx, g.ld2, g.fs2, g.fs2,
+                      T(1.0 / npts), W2<T>(p), W2<T>(p), p->tw[0], mxs, xr, s);
+  if (rc != 1) return rc;
   OpXNS2D<T, kXThr> ox;
   ox.in = W2<T>(p);
   ox.out = W2<T>(p);
-  ox.maxima =
-      want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
+  ox.maxima = mxs;
   ox.beta = (T)beta;
-  const double npts = (double)p->nx * p->ny;
-  note(K_XPASS, 5.0 * p->ny * g.mx * sizeof(cpx<T>));
   return launch_xpass<T>(ox, p->nx, p->ny, 4, 1, g.mx, g.ld2, g.mx, g.ld2,
                          g.fs2, g.fs2, T(1.0 / npts), p->tw[0], s);
 }
@@ -538,7 +636,7 @@ int ns3d_front(fsx_plan_s* p, const void* v, int has_f, double fcor,
   op.with_b = strat;
   note(K_ZINV, (double)g.ym.nk() * g.mx *
                    ((nin - 3.0) * g.zm.nk() + (double)nin * p->nz) * sizeof(cpx<T>));
-  int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, nin, p->tw[2], s);
+  int rc = launch_strided<T, +1>(op, p->nz, g.ym.nk(), g.mx, nin, p->tw[2], s, true);
   if (rc) return rc;
   OpYInv3<T> oy;
   oy.in = W1<T>(p);
@@ -556,6 +654,17 @@ int ns3d_front(fsx_plan_s* p, const void* v, int has_f, double fcor,
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
   const double npts = (double)p->nx * p->ny * p->nz;
   note(K_XPASS, (double)(nin + nout) * nrows(p) * g.mx * sizeof(cpx<T>));
+  if (strat) {
+    XRStrat<T> xr;
+    rc = launch_xrow<T>(p->nx, nrows(p), g.mx, g.ld2, g.mx, g.ld2, g.fs2, g.fs2,
+                        T(1.0 / npts), W2<T>(p), W2<T>(p), p->tw[0], mx, xr, s);
+    if (rc != 1) return rc;
+  } else {
+    XRNS3D<T> xr;
+    rc = launch_xrow<T>(p->nx, nrows(p), g.mx, g.ld2, g.mx, g.ld2, g.fs2, g.fs2,
+                        T(1.0 / npts), W2<T>(p), W2<T>(p), p->tw[0], mx, xr, s);
+    if (rc != 1) return rc;
+  }
   if (strat) {
     OpXStrat<T, kXThr> ox;
     ox.in = W2<T>(p);
@@ -683,7 +792,7 @@ int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
   op.rk = make_rk<T>(p, scheme, stage, 1, S, tmp, t1, nullptr, 0, 0, visc);
   note(K_ZFWD, (double)g.mx * p->ny * sizeof(cpx<T>) +
                    rk_bytes(p, scheme, stage, 1, sizeof(cpx<T>)));
-  return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s);
+  return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s, true);
 }
 
 template <typename T>
@@ -724,7 +833,7 @@ int do_ns3d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
                      visc);
   note(K_ZFWD, 3.0 * g.ym.nk() * g.mx * p->nz * sizeof(cpx<T>) +
                    rk_bytes(p, scheme, stage, 3, sizeof(cpx<T>)));
-  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, 3, p->tw[2], s);
+  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, 3, p->tw[2], s, true);
 }
 
 template <typename T>
@@ -798,7 +907,7 @@ int do_strat(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp, void* t1,
                                : rk_bytes(p, scheme, stage, 4, sizeof(cpx<T>));
   note(K_ZFWD, 6.0 * g.ym.nk() * g.mx * p->nz * sizeof(cpx<T>) +
                    2.0 * kept * sizeof(cpx<T>) + rkb);
-  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, 6, p->tw[2], s);
+  return launch_strided<T, -1>(op, p->nz, g.ym.nk(), g.mx, 6, p->tw[2], s, true);
 }
 
 }  // namespace

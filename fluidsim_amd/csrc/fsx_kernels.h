@@ -32,6 +32,7 @@ struct StridedDesc {
   int nouter;     // number of independent planes
   int nlines;     // lines per plane
   int nf;         // fields transformed one after the other
+  int kio;        // the pass reads or writes K rows (tiles remapped per XCD)
   const cpx<T>* tw;  // exp(-2 pi i m / N), m < N (copied to LDS)
 };
 
@@ -154,6 +155,21 @@ __device__ __forceinline__ bool last_slot(int e, int logN, int logL, int& line,
 }
 
 // ---------------------------------------------------------------- strided
+// Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
+// XCD b mod FSX_NXCD), each XCD with its own L2.  In passes over K rows
+// (nkx = nx/2 + 1 complex values, not line-aligned) neighbouring line chunks
+// share the cache lines that straddle their boundary, so there consecutive
+// tiles are given to the same XCD: XCD x takes the x-th contiguous range.
+#ifndef FSX_NXCD
+#define FSX_NXCD 8
+#endif
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  if (FSX_NXCD <= 1 || nb < FSX_NXCD) return b;
+  const int x = b % FSX_NXCD, local = b / FSX_NXCD;
+  const int q = nb / FSX_NXCD, rem = nb % FSX_NXCD;
+  return x < rem ? x * (q + 1) + local : rem * (q + 1) + (x - rem) * q + local;
+}
+
 template <typename T, int DIR, int EPT, int kT, class Op>
 __global__ __launch_bounds__(kT) void k_strided(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -163,8 +179,9 @@ __global__ __launch_bounds__(kT) void k_strided(Op op) {
   for (int i = threadIdx.x; i < d.N; i += kT) s_tw[i] = d.tw[i];
   // (tile_fft's first barrier precedes every twiddle read)
   const int nchunks = (d.nlines + d.L - 1) >> d.logL;
-  const int o = blockIdx.x / nchunks;
-  const int line0 = (blockIdx.x - o * nchunks) << d.logL;
+  const int b = d.kio ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int o = b / nchunks;
+  const int line0 = (b - o * nchunks) << d.logL;
   op.template run<DIR, EPT, kT>(buf, s_tw, o, line0);
 }
 

@@ -1,6 +1,7 @@
 """Summarise rocprofv3 result databases (rocpd SQLite) as text.
 
     python -m fluidsim_amd.rocpd_summary kernels DB [DB_PMC ...]
+    python -m fluidsim_amd.rocpd_summary traffic FETCH_DB WRITE_DB COPY_MIB [KEY_PREFIX]
 
 ``kernels``: per kernel name, calls, average / total duration (from the
 kernel trace); with PMC databases, the average counter value per dispatch of
@@ -54,7 +55,57 @@ def pmc_table(db):
     return agg
 
 
+def dispatch_values(db, counter):
+    """(kernel name, duration ns, value, grid size) per dispatch."""
+    con = sqlite3.connect(db)
+    return con.execute(
+        "select kernel_name, duration, value, grid_size from counters_collection "
+        "where counter_name = ? order by dispatch_id", (counter,)
+    ).fetchall()
+
+
+def traffic(fetch_db, write_db, copy_mib, prefix=""):
+    """HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
+    each scaled by what the counter reads for the device-to-device copy of
+    COPY_MIB MiB (the copyBuffer dispatch that reads and writes COPY_MIB MiB):
+    factor = true KiB / counted KiB."""
+    import json
+
+    true_kib = copy_mib * 1024.0
+    out = {}
+    factors = {}
+    for counter, db in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
+        rows = dispatch_values(db, counter)
+        copies = [r for r in rows if "copyBuffer" in r[0]]
+        if not copies:
+            raise SystemExit(f"no copyBuffer dispatch in {db}")
+        cal = max(copies, key=lambda r: r[1])  # the large calibration copy
+        factors[counter] = true_kib / cal[2]
+        agg = defaultdict(list)
+        for name, dur, val, grid in rows:
+            agg[short_name(name)].append(val)
+        for name, vals in agg.items():
+            out.setdefault(name, {})[counter] = sum(vals) / len(vals)
+    print(f"calibration: {copy_mib} MiB device copy; FETCH_SIZE factor "
+          f"{factors['FETCH_SIZE']:.3f}, WRITE_SIZE factor {factors['WRITE_SIZE']:.3f}")
+    print(f"{'kernel':60s} {'FETCH KiB':>14s} {'WRITE KiB':>14s} {'HBM MB/launch':>14s}")
+    res = {}
+    for name, c in sorted(out.items(), key=lambda kv: -sum(kv[1].values())):
+        if "FETCH_SIZE" not in c or "WRITE_SIZE" not in c:
+            continue
+        b = 1024.0 * (c["FETCH_SIZE"] * factors["FETCH_SIZE"]
+                      + c["WRITE_SIZE"] * factors["WRITE_SIZE"])
+        res[name] = b
+        print(f"{name[:60]:60s} {c['FETCH_SIZE']:14.1f} {c['WRITE_SIZE']:14.1f} {b / 1e6:14.1f}")
+    if prefix:
+        print(json.dumps({prefix + k: v for k, v in res.items()}))
+    return res
+
+
 def main(argv):
+    if len(argv) >= 4 and argv[0] == "traffic":
+        traffic(argv[1], argv[2], float(argv[3]), argv[4] if len(argv) > 4 else "")
+        return 0
     if len(argv) < 2 or argv[0] != "kernels":
         print(__doc__)
         return 2
