@@ -101,6 +101,25 @@ def cpu_baseline(n_sample, n_metric, steps=1):
     }
 
 
+def measured_copy_bandwidth(torch, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of 1 GiB (read + write), timed with HIP events:
+    the practical HBM ceiling the kernels are also compared with."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def read_traffic(kernel_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -138,6 +157,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n, prec = args.n, (4 if args.precision == "float32" else 8)
+    copy_gbps = measured_copy_bandwidth(torch)
     if args.copy_mb:
         src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -219,11 +239,14 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
             "traffic": traffic,
+            "copy_peak_measured": copy_gbps,
+            "frac_of_copy": achieved / copy_gbps,
         },
         "step_roofline": {
             "B_alg_GB_per_step": balg / 1e9,
             "achieved_GBps": balg * steps_per_s / 1e9,
             "frac": balg * steps_per_s / 1e9 / PEAK_HBM_GBPS,
+            "frac_of_copy": balg * steps_per_s / 1e9 / copy_gbps,
         },
         "kernels": kernels,
     }

@@ -154,6 +154,16 @@ __device__ __forceinline__ bool last_slot(int e, int logN, int logL, int& line,
   return true;
 }
 
+// Elements per thread of the multi-transform z-passes (tile = kThr x EPT /
+// div): the curl pass stashes 3 and the Runge-Kutta passes 3-4 transforms
+// in registers, so their tiles are halved by default.
+#ifndef FSX_ZCURL_EPTDIV
+#define FSX_ZCURL_EPTDIV 2
+#endif
+#ifndef FSX_ZRK_EPTDIV
+#define FSX_ZRK_EPTDIV 2
+#endif
+
 // ---------------------------------------------------------------- strided
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
 // XCD b mod FSX_NXCD), each XCD with its own L2.  In passes over K rows
@@ -334,7 +344,7 @@ struct OpYFwd2 : PlainRun<T, OpYFwd2<T>> {
 // each input read from HBM once; outputs 0-2 are v, 3-5 omega.
 template <typename T>
 struct OpZInvCurl {
-  static constexpr int kEptDiv = 2;  // register stashes: half-size tiles
+  static constexpr int kEptDiv = FSX_ZCURL_EPTDIV;  // register stashes: half-size tiles
   static constexpr bool kFieldPerBlock = false;
   StridedDesc<T> d;
   const cpx<T>* in;  // 3 fields
@@ -482,7 +492,7 @@ struct OpYInv2NS {
 // (rk_point); the tendency never goes to HBM.
 template <typename T>
 struct OpZFwdRK3 {
-  static constexpr int kEptDiv = 2;
+  static constexpr int kEptDiv = FSX_ZRK_EPTDIV;
   static constexpr bool kFieldPerBlock = false;
   StridedDesc<T> d;
   const cpx<T>* in;  // W1, 3 fields
@@ -536,7 +546,7 @@ struct OpZFwdRK3 {
 // projection acts on N_v only (rk_point).
 template <typename T>
 struct OpZFwdRKStrat {
-  static constexpr int kEptDiv = 2;
+  static constexpr int kEptDiv = FSX_ZRK_EPTDIV;
   static constexpr bool kFieldPerBlock = false;
   StridedDesc<T> d;
   const cpx<T>* in;  // W1, 6 fields
