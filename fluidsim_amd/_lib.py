@@ -71,6 +71,15 @@ SIGNATURES = {
         ctypes.POINTER(c_double), P,
     ],
     "fsx_spectrum": [P, P, P, P, c_int, c_int, ctypes.POINTER(c_double), P],
+    "fsx_forcing_setup": [
+        P, c_int, c_int, c_int, c_int, c_int, P, P, c_int, c_int,
+    ],
+    "fsx_forcing_set_pair": [P, P, P, P],
+    "fsx_forcing_set_coarse": [P, P, P],
+    "fsx_forcing_enable": [P, c_int],
+    "fsx_forcing_step": [P, c_double, c_double, c_int],
+    "fsx_forcing_normalize": [P, P, P],
+    "fsx_forcing_add": [P, P, c_int, c_double, P],
     "fsx_timing_enable": [P, c_int],
     "fsx_timing_read": [
         P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),

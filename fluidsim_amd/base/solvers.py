@@ -32,6 +32,8 @@ class InfoSolverPseudoSpectral(InfoSolver):
                         "InitFieldsBase")
         self._set_class("Output", "fluidsim_amd.base.output",
                         "OutputBasePseudoSpectral")
+        self._set_class("Forcing", "fluidsim_amd.base.forcing",
+                        "ForcingBasePseudoSpectral")
 
 
 class InfoSolverPseudoSpectral3D(InfoSolverPseudoSpectral):
@@ -79,24 +81,30 @@ class SimulBasePseudoSpectral:
                 "nu_m4": 0.0,
             }
         )
-        params._set_child("forcing", attribs={"enable": False, "type": ""})
 
     def __init__(self, params):
         self.params = params
         self.info_solver = self.InfoSolver()
         self.info_solver.complete_with_classes()
         dict_classes = self.info_solver.import_classes()
-        if getattr(params.forcing, "enable", False):
-            raise NotImplementedError(
-                "forcing is not available in the MI355X engine yet"
-            )
-        self.is_forcing_enabled = False
         self.oper = dict_classes["Operators"](params=params)
         self.output = dict_classes["Output"](self)
         self.state = dict_classes["State"](self)
         self.time_stepping = dict_classes["TimeStepping"](self)
         self.init_fields = dict_classes["InitFields"](self)
         self.init_fields()
+        # other classes with a params.<task>.enable switch
+        # (fluidsim/base/solvers/base.py:190-213): the forcing
+        simple = ("Operators", "Output", "State", "TimeStepping", "InitFields")
+        for role, cls in dict_classes.items():
+            if role in simple:
+                continue
+            name_task = cls._name_task
+            setattr(self, f"is_{name_task}_enabled", False)
+            params_cls = getattr(params, name_task, None)
+            if params_cls is not None and params_cls.enable:
+                setattr(self, f"is_{name_task}_enabled", True)
+                setattr(self, name_task, cls(self))
         self.output.post_init()
 
     def __enter__(self):

@@ -173,6 +173,10 @@ class TimeSteppingPseudoSpectral:
                 f"Stop signal ({name}) received so _has_to_stop set to True"
             )
             self._has_to_stop = True
+        if getattr(self.sim, "is_forcing_enabled", False):
+            # base.py:212-213 (the normalisation itself runs on the device
+            # inside the first stage, after the CFL update)
+            self.sim.forcing.compute()
         self.sim.output.one_time_step()
         self.one_time_step_computation()
         self.t += self.deltat
@@ -225,12 +229,16 @@ class TimeSteppingPseudoSpectral:
         tmp = self._state_spect_tmp.data
         t1 = self._state_spect_tmp1.data
         self._push_dt()
+        maker = (sim.forcing.forcing_maker
+                 if getattr(sim, "is_forcing_enabled", False) else None)
         if self.fused and hasattr(sim, "_engine_rk_stage"):
             cfl = float(self.CFL) if p.USE_CFL else 0.0
             for stage in range(self._scheme):
                 sim._engine_rk_stage(self._scheme, stage, S, tmp, t1, self._visc,
                                      cfl if stage == 0 else 0.0,
                                      float(self.deltat_max))
+            if maker is not None:
+                maker._pending = False  # consumed by stage 0
             return
         N = self._tendencies.data
         nvar = S.shape[0]
@@ -242,6 +250,9 @@ class TimeSteppingPseudoSpectral:
         if p.USE_CFL:
             _lib.call("fsx_cfl_update", plan, sim._cfl_ncomp, float(self.CFL),
                       float(self.deltat_max), s)
+        if maker is not None and getattr(maker, "_pending", False):
+            _lib.call("fsx_forcing_normalize", plan, _lib.dptr(S), s)
+            maker._pending = False
         for stage in range(self._scheme):
             if stage > 0:
                 sim._engine_tendency(t1, N, want_maxima=False)

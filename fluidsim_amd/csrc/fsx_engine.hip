@@ -18,6 +18,7 @@
 #include "fsx_kernels.h"
 #include "fsx_pointwise.h"
 #include "fsx_rows.h"
+#include "fsx_forcing.h"
 
 using namespace fsx;
 
@@ -81,6 +82,19 @@ struct fsx_plan_s {
   double* d_scal;  // [0..3] maxima bits, [4] dt, [5] nan flag, [8..15] sums
   double* d_spec;
   int spec_cap;
+  // coarse forcing box (fsx_forcing_*)
+  bool force_on, force_pending;
+  void* d_fc;
+  long long fvs;
+  int fnvar, fnyc, fnzc, fnkxc;
+  int* d_fmodes;
+  double* d_fw;
+  void* d_ff0;
+  void* d_ff1;
+  int fnmodes;
+  int fkey_kind, fkey_var;
+  double fcoef, frate;
+  int froot;
 };
 
 namespace {
@@ -715,7 +729,58 @@ RKArgs<T> make_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S,
   a.visc.num4 = visc[3];
   a.box = make_box(p, true);
   a.nanflag = reinterpret_cast<unsigned int*>(p->d_scal + 5);
+  // the Runge-Kutta stages add the forcing; tendency-only calls do not
+  const bool forced = p->force_on && p->d_fc && stage >= 0;
+  a.fc = forced ? reinterpret_cast<const cpx<T>*>(p->d_fc) : nullptr;
+  a.fvs = p->fvs;
+  a.fnyc = p->fnyc;
+  a.fnzc = p->fnzc;
+  a.fnkxc = p->fnkxc;
   return a;
+}
+
+// Normalised forcing of the step (k_forcing_norm) on the state S.
+template <typename T>
+int forcing_normalize(fsx_plan_s* p, const void* S, hipStream_t s) {
+  if (!p->d_fc || !p->d_fmodes || p->fnmodes <= 0)
+    return fail(FSX_ERR_ARG, "forcing box not set up (fsx_forcing_setup)");
+  ForcingArgs<T> f;
+  f.S = reinterpret_cast<const cpx<T>*>(S);
+  f.ke = (long long)kelems(p);
+  f.fc = reinterpret_cast<cpx<T>*>(p->d_fc);
+  f.fvs = p->fvs;
+  f.nvar = p->fnvar;
+  f.modes = p->d_fmodes;
+  f.w = p->d_fw;
+  f.f0 = reinterpret_cast<const cpx<double>*>(p->d_ff0);
+  f.f1 = reinterpret_cast<const cpx<double>*>(p->d_ff1);
+  f.nmodes = p->fnmodes;
+  f.nyc = p->fnyc;
+  f.nzc = p->fnzc;
+  f.nkxc = p->fnkxc;
+  f.ny = p->ny;
+  f.nz = p->dim == 3 ? p->nz : 1;
+  f.nkx = p->nkx;
+  f.dkx = p->dkx;
+  f.dky = p->dky;
+  f.key_kind = p->fkey_kind;
+  f.key_var = p->fkey_var;
+  f.coef = p->fcoef;
+  f.rate = p->frate;
+  f.which_root = p->froot;
+  f.dt = p->d_scal + 4;
+  hipLaunchKernelGGL(k_forcing_norm<T>, dim3(1), dim3(256), 0, s, f);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+// Stage 0 of a fused step: the pending normalisation runs after the CFL
+// update (the dt of the step) and before the stage algebra, on S.
+template <typename T>
+int forcing_stage0(fsx_plan_s* p, int stage, const void* S, hipStream_t s) {
+  if (stage != 0 || !p->force_pending) return 0;
+  p->force_pending = false;
+  return forcing_normalize<T>(p, S, s);
 }
 
 // algorithmic bytes of the state streams of one Runge-Kutta stage
@@ -781,6 +846,7 @@ int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
   rc = ns2d_front<T>(p, stage == 0 ? S : t1, beta, use_cfl, s);
   if (rc) return rc;
   if (use_cfl && (rc = cfl_end(p, 2, cfl, dt_max, s))) return rc;
+  if ((rc = forcing_stage0<T>(p, stage, S, s))) return rc;
   const Geo g = geo<T>(p, true);
   OpYFwd2RK<T> op;
   op.in = W2<T>(p);
@@ -820,6 +886,7 @@ int do_ns3d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
     if (rc) return rc;
   }
   if (use_cfl && (rc = cfl_end(p, 3, cfl, dt_max, s))) return rc;
+  if ((rc = forcing_stage0<T>(p, stage, S, s))) return rc;
   OpZFwdRK3<T> op;
   op.in = W1<T>(p);
   op.ifs = g.fs1;
@@ -885,6 +952,7 @@ int do_strat(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp, void* t1,
     if (rc) return rc;
   }
   if (use_cfl && (rc = cfl_end(p, 3, cfl, dt_max, s))) return rc;
+  if ((rc = forcing_stage0<T>(p, stage, S, s))) return rc;
   OpZFwdRKStrat<T> op;
   op.in = W1<T>(p);
   op.ifs = g.fs1;
@@ -1053,6 +1121,11 @@ int fsx_plan_destroy(fsx_plan p) {
   if (p->d_mask) (void)hipFree(p->d_mask);
   if (p->d_scal) (void)hipFree(p->d_scal);
   if (p->d_spec) (void)hipFree(p->d_spec);
+  if (p->d_fc) (void)hipFree(p->d_fc);
+  if (p->d_fmodes) (void)hipFree(p->d_fmodes);
+  if (p->d_fw) (void)hipFree(p->d_fw);
+  if (p->d_ff0) (void)hipFree(p->d_ff0);
+  if (p->d_ff1) (void)hipFree(p->d_ff1);
   delete p;
   return 0;
 }
@@ -1335,6 +1408,140 @@ int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
   FSX_HIP(hipMemcpyAsync(host_out, p->d_spec, nbins * sizeof(double),
                          hipMemcpyDeviceToHost, s));
   FSX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsx_forcing_setup(fsx_plan p, int nyc, int nzc, int nkxc, int nvar,
+                      int nmodes, const int* modes, const double* weights,
+                      int key_kind, int key_var) {
+  if (!p || nvar < 1 || nvar > 4 || nyc < 2 || nkxc < 2 || nmodes < 0 ||
+      (nmodes > 0 && (!modes || !weights)))
+    return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim == 2 && nzc != 1) return fail(FSX_ERR_ARG, "2D forcing box needs nzc = 1");
+  if (p->dim == 3 && nzc < 2) return fail(FSX_ERR_ARG, "3D forcing box needs nzc >= 2");
+  if (nyc > p->ny || nkxc > p->nkx || (p->dim == 3 && nzc > p->nz))
+    return fail(FSX_ERR_ARG, "forcing box larger than the grid");
+  if (key_kind != FKEY_VAR && key_kind != FKEY_VT) return fail(FSX_ERR_ARG, "bad key kind");
+  if (key_kind == FKEY_VT && (p->dim != 3 || nvar < 3))
+    return fail(FSX_ERR_ARG, "vt forcing needs a 3D plan and 3 velocity fields");
+  if (key_kind == FKEY_VAR && (key_var < 0 || key_var >= nvar))
+    return fail(FSX_ERR_ARG, "bad forced variable");
+  const long long fvs = (long long)nzc * nyc * nkxc;
+  for (int m = 0; m < nmodes; ++m)
+    if (modes[m] < 0 || modes[m] >= fvs) return fail(FSX_ERR_ARG, "forced mode outside the box");
+  const size_t csz = 2 * (size_t)p->prec;
+  void* fc = nullptr;
+  int* dm = nullptr;
+  double* dw = nullptr;
+  void *f0 = nullptr, *f1 = nullptr;
+  hipError_t e = hipMalloc(&fc, fvs * nvar * csz);
+  if (e == hipSuccess) e = hipMemset(fc, 0, fvs * nvar * csz);
+  const size_t nm = nmodes > 0 ? nmodes : 1;
+  if (e == hipSuccess) e = hipMalloc((void**)&dm, nm * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&dw, nm * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&f0, nm * 2 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&f1, nm * 2 * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(f0, 0, nm * 2 * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(f1, 0, nm * 2 * sizeof(double));
+  if (e == hipSuccess && nmodes > 0)
+    e = hipMemcpy(dm, modes, nmodes * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nmodes > 0)
+    e = hipMemcpy(dw, weights, nmodes * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    for (void* q : {fc, (void*)dm, (void*)dw, f0, f1})
+      if (q) (void)hipFree(q);
+    return fail(FSX_ERR_HIP, std::string("forcing setup: ") + hipGetErrorString(e));
+  }
+  for (void* q : {p->d_fc, (void*)p->d_fmodes, (void*)p->d_fw, p->d_ff0, p->d_ff1})
+    if (q) (void)hipFree(q);
+  p->d_fc = fc;
+  p->d_fmodes = dm;
+  p->d_fw = dw;
+  p->d_ff0 = f0;
+  p->d_ff1 = f1;
+  p->fvs = fvs;
+  p->fnvar = nvar;
+  p->fnyc = nyc;
+  p->fnzc = nzc;
+  p->fnkxc = nkxc;
+  p->fnmodes = nmodes;
+  p->fkey_kind = key_kind;
+  p->fkey_var = key_var;
+  p->force_pending = false;
+  return 0;
+}
+
+int fsx_forcing_set_pair(fsx_plan p, const double* f0, const double* f1,
+                         void* stream) {
+  if (!p || !f0 || !f1) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->d_ff0) return fail(FSX_ERR_ARG, "forcing box not set up (fsx_forcing_setup)");
+  // earlier steps may still read the arrays on the stream
+  FSX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  const size_t b = (size_t)p->fnmodes * 2 * sizeof(double);
+  if (b) {
+    FSX_HIP(hipMemcpy(p->d_ff0, f0, b, hipMemcpyHostToDevice));
+    FSX_HIP(hipMemcpy(p->d_ff1, f1, b, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int fsx_forcing_set_coarse(fsx_plan p, const void* fc, void* stream) {
+  if (!p || !fc) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->d_fc) return fail(FSX_ERR_ARG, "forcing box not set up (fsx_forcing_setup)");
+  FSX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  FSX_HIP(hipMemcpy(p->d_fc, fc, p->fvs * p->fnvar * 2 * (size_t)p->prec,
+                    hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fsx_forcing_enable(fsx_plan p, int on) {
+  if (!p) return fail(FSX_ERR_ARG, "bad argument");
+  if (on && !p->d_fc) return fail(FSX_ERR_ARG, "forcing box not set up (fsx_forcing_setup)");
+  p->force_on = on != 0;
+  return 0;
+}
+
+int fsx_forcing_step(fsx_plan p, double coef, double forcing_rate, int which_root) {
+  if (!p) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->d_fc || !p->d_fmodes || p->fnmodes <= 0)
+    return fail(FSX_ERR_ARG, "forcing box not set up (fsx_forcing_setup)");
+  if (which_root < 0 || which_root > 3) return fail(FSX_ERR_ARG, "bad root choice");
+  p->fcoef = coef;
+  p->frate = forcing_rate;
+  p->froot = which_root;
+  p->force_pending = true;
+  return 0;
+}
+
+int fsx_forcing_normalize(fsx_plan p, const void* S, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !S) return fail(FSX_ERR_ARG, "bad argument");
+  p->force_pending = false;
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? forcing_normalize<float>(p, S, s)
+                      : forcing_normalize<double>(p, S, s);
+}
+
+int fsx_forcing_add(fsx_plan p, void* out_fft, int nvar, double scale, void* stream) {
+  if (!p || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->d_fc) return fail(FSX_ERR_ARG, "forcing box not set up (fsx_forcing_setup)");
+  if (nvar < 1 || nvar > p->fnvar) return fail(FSX_ERR_ARG, "bad number of fields");
+  hipStream_t s = (hipStream_t)stream;
+  const long long ke = (long long)kelems(p);
+  const int nz = p->dim == 3 ? p->nz : 1;
+  const long long n = p->fvs * nvar;
+  const unsigned blocks = (unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_forcing_add<float>, dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<cpx<float>*>(out_fft), ke,
+                       reinterpret_cast<const cpx<float>*>(p->d_fc), p->fvs, nvar,
+                       p->fnyc, p->fnzc, p->fnkxc, p->ny, nz, p->nkx, (float)scale);
+  else
+    hipLaunchKernelGGL(k_forcing_add<double>, dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<cpx<double>*>(out_fft), ke,
+                       reinterpret_cast<const cpx<double>*>(p->d_fc), p->fvs, nvar,
+                       p->fnyc, p->fnzc, p->fnkxc, p->ny, nz, p->nkx, scale);
+  FSX_HIP(hipGetLastError());
   return 0;
 }
 

@@ -109,7 +109,25 @@ struct RKArgs {
   Visc visc;
   Box box;
   unsigned int* nanflag;
+  // coarse forcing box added to the tendency before the projection (null:
+  // no forcing): fc[v * fvs + (cz * fnyc + cy) * fnkxc + cx]
+  const cpx<T>* fc;
+  long long fvs;
+  int fnyc, fnzc, fnkxc;
 };
+
+// Index in the coarse forcing box of the mode (ikx, iky, ikz) (signed ky,
+// kz), or -1 outside: the coarse grid of n_c points holds signed
+// wavenumbers -n_c/2 < k <= n_c/2 (put_coarse_array_in_array_fft,
+// fluidsim/operators/operators2d.py:525-600).
+__device__ __forceinline__ int coarse_index(int ikx, int iky, int ikz, int nkxc,
+                                            int nyc, int nzc) {
+  if (ikx >= nkxc || iky <= -nyc / 2 || iky > nyc / 2) return -1;
+  if (nzc > 1 && (ikz <= -nzc / 2 || ikz > nzc / 2)) return -1;
+  const int cy = iky >= 0 ? iky : iky + nyc;
+  const int cz = nzc > 1 ? (ikz >= 0 ? ikz : ikz + nzc) : 0;
+  return (cz * nyc + cy) * nkxc + ikx;
+}
 
 // Integrating-factor Runge-Kutta update of one mode
 // (fluidsim/base/time_stepping/pseudo_spect.py:51-68, 466-517, 898-984):
@@ -124,6 +142,14 @@ __device__ __forceinline__ void rk_point(const RKArgs<T>& a, double dt,
   const T e = T(exp(-dt * sig));
   const T e2 = T(exp(-dt / 2 * sig));
   const T c6 = T(dt / 6), c3 = T(dt / 3), c2 = T(dt / 2), c1 = T(dt);
+  if (a.fc) {
+    // tendencies += forcing (ns2d/solver.py:190-191, ns3d/solver.py:243-244)
+    const int c = coarse_index(ikx, iky, ikz, a.fnkxc, a.fnyc, a.fnzc);
+    if (c >= 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) N[v] = N[v] + a.fc[v * a.fvs + c];
+    }
+  }
   if constexpr (NV >= 3) {
     if (a.project != PROJ_NONE) {
       project3(N[0], N[1], N[2], a.project, ikx, iky, ikz, bx);

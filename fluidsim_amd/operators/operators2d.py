@@ -94,6 +94,12 @@ class OperatorsPseudoSpectral2D(OperatorsBase):
     def _K2_host(self):
         return self.K2
 
+    @property
+    def deltak(self):
+        """Largest of the deltak_i (fluidsim/base/forcing/specific.py
+        :178-182 uses it for the forcing wavenumbers)."""
+        return max(self.deltakx, self.deltaky)
+
     @cached_property
     def kx(self):
         return self.deltakx * np.arange(self.nx // 2 + 1, dtype=np.float64)

@@ -99,6 +99,12 @@ class OperatorsPseudoSpectral3D(OperatorsBase):
     def _K2_host(self):
         return self.K2
 
+    @property
+    def deltak(self):
+        """Largest of the deltak_i (fluidsim/base/forcing/specific.py
+        :178-182 uses it for the forcing wavenumbers)."""
+        return max(self.deltakx, self.deltaky, self.deltakz)
+
     @cached_property
     def kx(self):
         return self.deltakx * np.arange(self.nx // 2 + 1, dtype=np.float64)

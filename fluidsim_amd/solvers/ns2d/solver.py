@@ -22,6 +22,7 @@ class InfoSolverNS2D(InfoSolverPseudoSpectral):
         self._set_class("State", package + ".state", "StateNS2D")
         self._set_class("InitFields", package + ".init_fields", "InitFieldsNS2D")
         self._set_class("Output", package + ".output", "OutputNS2D")
+        self._set_class("Forcing", "fluidsim_amd.base.forcing", "ForcingNS2D")
 
 
 class Simul(SimulBasePseudoSpectral):
@@ -56,4 +57,8 @@ class Simul(SimulBasePseudoSpectral):
             # modes outside the kept box are never written by the engine
             tend.initialize(0.0)
         self._engine_tendency(state_spect.data, tend.data)
+        if self.is_forcing_enabled:
+            # solver.py:190-191, after the dealiasing
+            _lib.call("fsx_forcing_add", self.oper.plan.handle, _lib.dptr(tend.data),
+                      1, 1.0, _lib.stream_ptr())
         return tend

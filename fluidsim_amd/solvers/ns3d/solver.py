@@ -24,6 +24,7 @@ class InfoSolverNS3D(InfoSolverPseudoSpectral3D):
         self._set_class("State", package + ".state", "StateNS3D")
         self._set_class("InitFields", package + ".init_fields", "InitFieldsNS3D")
         self._set_class("Output", package + ".output", "OutputNS3D")
+        self._set_class("Forcing", "fluidsim_amd.base.forcing", "ForcingNS3D")
 
 
 class Simul(SimulBasePseudoSpectral):
@@ -87,5 +88,9 @@ class Simul(SimulBasePseudoSpectral):
         if old is not None:
             tend.initialize(0.0)
         self._engine_tendency(state_spect.data, tend.data)
+        if self.is_forcing_enabled:
+            # solver.py:243-244, before the projection
+            _lib.call("fsx_forcing_add", self.oper.plan.handle, _lib.dptr(tend.data),
+                      tend.nvar, 1.0, _lib.stream_ptr())
         self.project_state_spect(tend)
         return tend

@@ -59,4 +59,10 @@ class Simul(SimulNS3D):
         if old is not None:
             tend.initialize(0.0)
         self._engine_tendency(state_spect.data, tend.data)
+        if self.is_forcing_enabled:
+            # strat/solver.py:209-213: forcing added before the projection
+            # (the engine tendency is projected: P(P N + F) = P(N + F))
+            _lib.call("fsx_forcing_add", self.oper.plan.handle, _lib.dptr(tend.data),
+                      tend.nvar, 1.0, _lib.stream_ptr())
+            self.project_state_spect(tend)
         return tend

@@ -185,6 +185,58 @@ int fsx_sums(fsx_plan p, int kind, const void* f0, const void* f1,
 int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
                  int nf, int nbins, double* host_out, void* stream);
 
+/* ---- forcing on a coarse spectral box ----------------------------------
+ * fluidsim computes its forcing on a coarse grid and adds the full-size
+ * copy to the tendencies at every stage
+ * (fluidsim/base/forcing/specific.py:135-328, put_forcingc_in_forcing
+ * 279-290; fluidsim/solvers/ns2d/solver.py:190-191,
+ * fluidsim/solvers/ns3d/solver.py:243-244).  The engine keeps the coarse
+ * box Fc (nvar fields of nzc x nyc x nkxc complex values of the plan
+ * precision, nzc = 1 in 2D) on the device; while enabled, every Runge-Kutta
+ * stage adds Fc to the tendency of the modes inside the box (before the
+ * projection), the tendency-only entry points do not.
+ *
+ * fsx_forcing_setup: box shape, the forced modes (linear indices into the
+ * box) with their Parseval weights, and the forced key: key_kind 0 = state
+ * variable key_var, 1 = toroidal velocity vt_fft (3D; vecfft_from_vtfft,
+ * fluidsim/operators/operators3d.py:961-992).  Fc starts at zero. */
+int fsx_forcing_setup(fsx_plan p, int nyc, int nzc, int nkxc, int nvar,
+                      int nmodes, const int* modes, const double* weights,
+                      int key_kind, int key_var);
+
+/* Raw random forcings f0, f1 of the key at the forced modes (host arrays of
+ * nmodes interleaved complex128), TimeCorrelatedRandomPseudoSpectral's
+ * forcing0 / forcing1 (fluidsim/base/forcing/specific.py:768-868).
+ * Synchronises the stream first. */
+int fsx_forcing_set_pair(fsx_plan p, const double* f0, const double* f1,
+                         void* stream);
+
+/* Whole coarse box from the host (stationary forcings; plan precision).
+ * Synchronises the stream first. */
+int fsx_forcing_set_coarse(fsx_plan p, const void* fc, void* stream);
+
+/* Whether the Runge-Kutta stages add Fc. */
+int fsx_forcing_enable(fsx_plan p, int on);
+
+/* Normalisation of the next step (NormalizedForcing.compute,
+ * fluidsim/base/forcing/specific.py:460-500, 587-726): with
+ * f_r = f1 - coef (f1 - f0), a = dt/2 sum_w |f_r|^2 and
+ * b = sum_w Re(conj(v) f_r) over the forced modes (v: the forced key of the
+ * state, dt: the device time step), Fc = R f_r with R the root of
+ * a R^2 + b R - forcing_rate = 0 chosen by which_root (0 minabs, 1 first,
+ * 2 second, 3 positive).  fsx_forcing_step records the parameters; the
+ * next stage-0 call of fsx_ns2d/ns3d/strat_rk_stage normalises on S after
+ * its CFL update.  fsx_forcing_normalize does it at once on S (unfused
+ * stepping: after fsx_cfl_update, before fsx_rk_stage). */
+int fsx_forcing_step(fsx_plan p, double coef, double forcing_rate, int which_root);
+int fsx_forcing_normalize(fsx_plan p, const void* S, void* stream);
+
+/* out[v] += scale * Fc[v] for v < nvar, at the full-grid modes of the box
+ * (put_coarse_array_in_array_fft, fluidsim/operators/operators2d.py
+ * :525-600): the forcing added to tendencies_nonlin, or, on a zeroed array,
+ * get_forcing(). */
+int fsx_forcing_add(fsx_plan p, void* out_fft, int nvar, double scale, void* stream);
+
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its
  * stream.  fsx_timing_read (sync) returns, per launch kind

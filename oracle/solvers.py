@@ -56,6 +56,12 @@ class _SolverBase:
         )
         self.t = 0.0
         self.it = 0
+        # forcing (oracle.forcing.TCRandomForcing or any object with
+        # compute(state_spect, t, dt) -> full forcing), computed once per
+        # step after the CFL update (base.py:208-214) and added to the
+        # tendencies at every stage
+        self.forcing = None
+        self._F = None
         f_d, f_d_hypo = self.compute_freq_diss()
         self.freq_lin = f_d + f_d_hypo
         self._dt_coefs = None
@@ -152,9 +158,11 @@ class _SolverBase:
             raise ValueError(f"nan at it = {self.it}, t = {self.t:.4f}")
 
     def one_time_step(self):
-        """base.py:208-244 (CFL first, then the step, then t, it)."""
+        """base.py:208-244 (CFL first, forcing, then the step, t, it)."""
         if self.USE_CFL:
             self.compute_time_increment_CLF()
+        if self.forcing is not None:
+            self._F = self.forcing.compute(self.state_spect, self.t, self.deltat)
         self.one_time_step_computation()
         self.t += self.deltat
         self.it += 1
@@ -201,6 +209,8 @@ class NS2D(_SolverBase):
         out = np.empty_like(state_spect)
         out[0] = oper.fft(Frot)
         oper.dealiasing(out[0])
+        if self._F is not None:
+            out += self._F  # solver.py:190-191, after the dealiasing
         return out
 
     def init_from_rotfft(self, rot_fft):
@@ -308,6 +318,8 @@ class NS3D(_SolverBase):
         out[0] = oper.fft(fx)
         out[1] = oper.fft(fy)
         out[2] = oper.fft(fz)
+        if self._F is not None:
+            out += self._F  # solver.py:243-244, before the projection
         self.project_state_spect(out)
         oper.dealiasing(out)
         return out
@@ -378,6 +390,8 @@ class NS3DStrat(NS3D):
         div_vb_fft = oper.div_vb_fft_from_vb(v[0], v[1], v[2], b)
         # compute_fb_fft (29-33)
         out[3] = -div_vb_fft - self.N**2 * state_spect[2]
+        if self._F is not None:
+            out += self._F  # strat/solver.py:209-213
         self.project_state_spect(out)
         oper.dealiasing(out)
         return out
