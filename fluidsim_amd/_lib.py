@@ -80,6 +80,9 @@ SIGNATURES = {
     "fsx_forcing_step": [P, c_double, c_double, c_int],
     "fsx_forcing_normalize": [P, P, P],
     "fsx_forcing_add": [P, P, c_int, c_double, P],
+    "fsx_strain_rate": [P, P, P, P],
+    "fsx_smagorinsky_phys": [P, P, c_double, P],
+    "fsx_smagorinsky_div": [P, P, P, c_int, P],
     "fsx_timing_enable": [P, c_int],
     "fsx_timing_read": [
         P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),

@@ -231,7 +231,13 @@ class TimeSteppingPseudoSpectral:
         self._push_dt()
         maker = (sim.forcing.forcing_maker
                  if getattr(sim, "is_forcing_enabled", False) else None)
-        if self.fused and hasattr(sim, "_engine_rk_stage"):
+        # the model term of a turbulence model needs the tendency array:
+        # separate tendency + stage kernels then
+        fused = self.fused and not (
+            getattr(sim, "is_turb_model_enabled", False)
+            and getattr(sim, "_turb_model_in_tendency", True)
+        )
+        if fused and hasattr(sim, "_engine_rk_stage"):
             cfl = float(self.CFL) if p.USE_CFL else 0.0
             for stage in range(self._scheme):
                 sim._engine_rk_stage(self._scheme, stage, S, tmp, t1, self._visc,

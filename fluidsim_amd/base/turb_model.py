@@ -1,0 +1,147 @@
+"""Turbulence models (fluidsim/base/turb_model/base.py:23-120,
+smagorinsky.py:18-110).
+
+``extend_simul_class(Simul, SmagorinskyModel)`` adds ``classes.TurbModel``
+to the solver info and ``params.turb_model`` (enable, type, smagorinsky.C);
+with ``params.turb_model.enable`` the Simul gets ``sim.turb_model`` and the
+ns3d tendencies include the model term before the projection
+(fluidsim/solvers/ns3d/solver.py:246-249).  The model runs on the device:
+strain rate, c2r, 2 C_nuT |S| S_ij, r2c and the divergence
+(fsx_strain_rate, fsx_ifft, fsx_smagorinsky_phys, fsx_fft,
+fsx_smagorinsky_div).
+"""
+
+import importlib
+from math import sqrt
+
+from .. import _lib
+from ..extend_simul import SimulExtender, extend_simul_class  # noqa: F401
+from ..setofvariables import SetOfVariables
+
+
+def modif_infosolver_turb_model(info_solver):
+    """base.py:23-37."""
+    if "TurbModel" not in info_solver.classes._tag_children:
+        info_solver.classes._set_child(
+            "TurbModel",
+            attribs={"class_name": "TurbModel",
+                     "module_name": "fluidsim_amd.base.turb_model"},
+        )
+    node = info_solver.classes.TurbModel
+    if "classes" not in node._tag_children:
+        node._set_child("classes")
+
+
+def _model_classes(info_solver):
+    node = info_solver.classes.TurbModel.classes
+    out = {}
+    for tag in node._tag_children:
+        child = getattr(node, tag)
+        out[tag] = getattr(importlib.import_module(child.module_name), child.class_name)
+    return out
+
+
+class TurbModel:
+    """base.py:40-75."""
+
+    _name_task = "turb_model"
+
+    @classmethod
+    def _complete_params_with_default(cls, params, info_solver):
+        params._set_child(cls._name_task, attribs={"enable": False, "type": ""})
+        for tag, klass in sorted(_model_classes(info_solver).items()):
+            klass._complete_params_with_default(params)
+
+    def __init__(self, sim):
+        self.type_model = sim.params.turb_model.type
+        dict_classes = _model_classes(sim.info_solver)
+        try:
+            cls_model = dict_classes[self.type_model]
+        except KeyError:
+            raise ValueError(
+                f"Wrong value ('{self.type_model}') of params.turb_model.type. "
+                f"It should be in {dict_classes.keys()}"
+            )
+        self._model = cls_model(sim)
+
+    def get_forcing(self, **kwargs):
+        return self._model.get_forcing(**kwargs)
+
+
+class SpecificTurbModel(SimulExtender):
+    """base.py:78-111."""
+
+    _module_name = "fluidsim_amd.base.turb_model"
+    tag = "specific_turb_model"
+
+    @classmethod
+    def get_modif_info_solver(cls):
+        def modif_info_solver(info_solver):
+            modif_infosolver_turb_model(info_solver)
+            info_solver.classes.TurbModel.classes._set_child(
+                cls.tag,
+                attribs={"module_name": cls._module_name, "class_name": cls.__name__},
+            )
+
+        return modif_info_solver
+
+    @classmethod
+    def complete_params_with_default(cls, params):
+        pass
+
+
+class SpecificTurbModelSpectral(SpecificTurbModel):
+    def __init__(self, sim):
+        self.sim = sim
+        self.forcing_fft = SetOfVariables(
+            like=sim.state.state_spect, info="forcing_fft", value=0.0
+        )
+
+
+class SmagorinskyModel(SpecificTurbModelSpectral):
+    """smagorinsky.py:18-110: f_i = 2i k_j fft(2 C_nuT |S| S_ij),
+    C_nuT = C (Lx / nx)^2 sqrt(2)."""
+
+    _module_name = "fluidsim_amd.base.turb_model"
+    tag = "smagorinsky"
+
+    @classmethod
+    def complete_params_with_default(cls, params):
+        params.turb_model._set_child(cls.tag, attribs={"C": 0.18})
+
+    def __init__(self, sim):
+        super().__init__(sim)
+        if sim.oper.dim != 3:
+            raise NotImplementedError("the Smagorinsky model is 3D (stress_tensor.py)")
+        C = sim.params.turb_model.smagorinsky.C
+        delta = sim.params.oper.Lx / sim.params.oper.nx
+        self.C_nu_T = C * delta**2 * sqrt(2)
+        oper = sim.oper
+        import torch
+
+        # workspaces: 6 K fields (S^, then g^) and 6 X fields (S, then g)
+        self._Sk = torch.empty((6,) + tuple(oper.shapeK), dtype=oper.dtype_complex,
+                               device="cuda")
+        self._Sx = torch.empty((6,) + tuple(oper.shapeX), dtype=oper.dtype_real,
+                               device="cuda")
+
+    def add_to(self, v_fft, out_fft, accumulate=True):
+        """out[0:3] (+)= model term of the velocity v_fft[0:3] (device)."""
+        h = self.sim.oper.plan.handle
+        s = _lib.stream_ptr()
+        _lib.call("fsx_strain_rate", h, _lib.dptr(v_fft), _lib.dptr(self._Sk), s)
+        _lib.call("fsx_ifft", h, _lib.dptr(self._Sk), _lib.dptr(self._Sx), 6, s)
+        _lib.call("fsx_smagorinsky_phys", h, _lib.dptr(self._Sx),
+                  2.0 * self.C_nu_T, s)
+        _lib.call("fsx_fft", h, _lib.dptr(self._Sx), _lib.dptr(self._Sk), 6, s)
+        _lib.call("fsx_smagorinsky_div", h, _lib.dptr(self._Sk), _lib.dptr(out_fft),
+                  int(accumulate), s)
+
+    def get_forcing(self, **kwargs):
+        """smagorinsky.py:52-110 (vx_fft, vy_fft, vz_fft as device tensors)."""
+        import torch
+
+        v = torch.stack([kwargs["vx_fft"], kwargs["vy_fft"], kwargs["vz_fft"]])
+        self.forcing_fft.initialize(0.0)
+        self.add_to(v.contiguous(), self.forcing_fft.data, accumulate=False)
+        return self.forcing_fft

@@ -19,6 +19,7 @@
 #include "fsx_pointwise.h"
 #include "fsx_rows.h"
 #include "fsx_forcing.h"
+#include "fsx_turb.h"
 
 using namespace fsx;
 
@@ -1541,6 +1542,72 @@ int fsx_forcing_add(fsx_plan p, void* out_fft, int nvar, double scale, void* str
                        reinterpret_cast<cpx<double>*>(out_fft), ke,
                        reinterpret_cast<const cpx<double>*>(p->d_fc), p->fvs, nvar,
                        p->fnyc, p->fnzc, p->fnkxc, p->ny, nz, p->nkx, scale);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+static unsigned grid_blocks(long long n) {
+  const long long b = (n + 255) / 256;
+  return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+static KGrid kgrid(const fsx_plan_s* p) {
+  KGrid g;
+  g.nz = p->nz;
+  g.ny = p->ny;
+  g.nkx = p->nkx;
+  g.dkx = p->dkx;
+  g.dky = p->dky;
+  g.dkz = p->dkz;
+  return g;
+}
+
+int fsx_strain_rate(fsx_plan p, const void* v_fft, void* S_fft, void* stream) {
+  if (!p || !v_fft || !S_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3) return fail(FSX_ERR_ARG, "the strain rate needs a 3D plan");
+  hipStream_t s = (hipStream_t)stream;
+  const long long ke = (long long)kelems(p);
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_strain_rate<float>, dim3(grid_blocks(ke)), dim3(256), 0, s,
+                       reinterpret_cast<const cpx<float>*>(v_fft),
+                       reinterpret_cast<cpx<float>*>(S_fft), ke, kgrid(p));
+  else
+    hipLaunchKernelGGL(k_strain_rate<double>, dim3(grid_blocks(ke)), dim3(256), 0, s,
+                       reinterpret_cast<const cpx<double>*>(v_fft),
+                       reinterpret_cast<cpx<double>*>(S_fft), ke, kgrid(p));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_smagorinsky_phys(fsx_plan p, void* S, double coef, void* stream) {
+  if (!p || !S) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3) return fail(FSX_ERR_ARG, "the Smagorinsky model needs a 3D plan");
+  hipStream_t s = (hipStream_t)stream;
+  const long long xe = (long long)xelems(p);
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_smag_phys<float>, dim3(grid_blocks(xe)), dim3(256), 0, s,
+                       reinterpret_cast<float*>(S), xe, (float)coef);
+  else
+    hipLaunchKernelGGL(k_smag_phys<double>, dim3(grid_blocks(xe)), dim3(256), 0, s,
+                       reinterpret_cast<double*>(S), xe, coef);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_smagorinsky_div(fsx_plan p, const void* g_fft, void* out_fft, int accumulate,
+                        void* stream) {
+  if (!p || !g_fft || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 3) return fail(FSX_ERR_ARG, "the Smagorinsky model needs a 3D plan");
+  hipStream_t s = (hipStream_t)stream;
+  const long long ke = (long long)kelems(p);
+  if (p->prec == 4)
+    hipLaunchKernelGGL(k_smag_div<float>, dim3(grid_blocks(ke)), dim3(256), 0, s,
+                       reinterpret_cast<const cpx<float>*>(g_fft),
+                       reinterpret_cast<cpx<float>*>(out_fft), ke, kgrid(p), accumulate);
+  else
+    hipLaunchKernelGGL(k_smag_div<double>, dim3(grid_blocks(ke)), dim3(256), 0, s,
+                       reinterpret_cast<const cpx<double>*>(g_fft),
+                       reinterpret_cast<cpx<double>*>(out_fft), ke, kgrid(p), accumulate);
   FSX_HIP(hipGetLastError());
   return 0;
 }

@@ -38,8 +38,12 @@ class InfoSolver(Parameters):
         return out
 
     def complete_with_classes(self):
-        """Let each class add its info (fluidsim's _complete_info_solver)."""
+        """Let each class add its info (fluidsim's _complete_info_solver),
+        then apply the modifications of the Simul extenders
+        (lib/fluidsim_core/info.py:60-75)."""
         for cls in self.import_classes().values():
             f = getattr(cls, "_complete_info_solver", None)
             if f is not None:
                 f(self)
+        for modif in getattr(self, "_modificators", ()):
+            modif(self)

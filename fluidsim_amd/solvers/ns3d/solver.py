@@ -59,6 +59,9 @@ class Simul(SimulBasePseudoSpectral):
         folded into the Runge-Kutta stage)."""
         self.oper.ns3d_tendency(v_fft, out_fft, f=self.params.f,
                                 want_maxima=want_maxima)
+        if getattr(self, "is_turb_model_enabled", False):
+            # solver.py:246-249: model term of the same state
+            self.turb_model._model.add_to(v_fft, out_fft)
 
     def _engine_rk_stage(self, scheme, stage, S, tmp, t1, visc, cfl, dt_max):
         """One fused Runge-Kutta stage on the device (fsx_ns3d_rk_stage)."""

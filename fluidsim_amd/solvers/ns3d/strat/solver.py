@@ -28,6 +28,8 @@ class InfoSolverNS3DStrat(InfoSolverNS3D):
 
 class Simul(SimulNS3D):
     InfoSolver = InfoSolverNS3DStrat
+    # strat/solver.py:138-216 adds no turbulence-model term
+    _turb_model_in_tendency = False
 
     @staticmethod
     def _complete_params_with_default(params):
@@ -52,7 +54,7 @@ class Simul(SimulNS3D):
                   _lib.stream_ptr())
 
     def tendencies_nonlin(self, state_spect=None, old=None):
-        """strat/solver.py:138-216 (no forcing)."""
+        """strat/solver.py:138-216."""
         if state_spect is None:
             state_spect = self.state.state_spect
         tend = SetOfVariables(like=self.state.state_spect) if old is None else old

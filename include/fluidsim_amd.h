@@ -237,6 +237,22 @@ int fsx_forcing_normalize(fsx_plan p, const void* S, void* stream);
  * get_forcing(). */
 int fsx_forcing_add(fsx_plan p, void* out_fft, int nvar, double scale, void* stream);
 
+/* ---- Smagorinsky turbulence model (3D plans) ----------------------------
+ * SmagorinskyModel.get_forcing (fluidsim/base/turb_model/smagorinsky.py
+ * :52-110, stress_tensor.py:15-44), in three pointwise steps around the
+ * plan's transforms; fields ordered (xx, yy, zz, yx, zx, zy):
+ *   fsx_strain_rate:      S_fft[6] = i (k_i v_j + k_j v_i) / 2 from v_fft[3];
+ *   fsx_ifft(S_fft -> S, 6 fields);
+ *   fsx_smagorinsky_phys: S[6] <- coef |S| S_ij in place, |S| = sqrt(S_ij S_ij)
+ *                         (coef = 2 C_nuT, C_nuT = C (Lx/nx)^2 sqrt(2));
+ *   fsx_fft(S -> g_fft, 6 fields);
+ *   fsx_smagorinsky_div:  out[3] (+)= 2i sum_j k_j g_fft_ij
+ *                         (accumulate != 0 adds to out). */
+int fsx_strain_rate(fsx_plan p, const void* v_fft, void* S_fft, void* stream);
+int fsx_smagorinsky_phys(fsx_plan p, void* S, double coef, void* stream);
+int fsx_smagorinsky_div(fsx_plan p, const void* g_fft, void* out_fft, int accumulate,
+                        void* stream);
+
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its
  * stream.  fsx_timing_read (sync) returns, per launch kind

@@ -62,6 +62,9 @@ class _SolverBase:
         # tendencies at every stage
         self.forcing = None
         self._F = None
+        # turbulence model (oracle.turb_model.Smagorinsky) evaluated on the
+        # state of each stage, added before the projection (3D)
+        self.turb_model = None
         f_d, f_d_hypo = self.compute_freq_diss()
         self.freq_lin = f_d + f_d_hypo
         self._dt_coefs = None
@@ -320,6 +323,11 @@ class NS3D(_SolverBase):
         out[2] = oper.fft(fz)
         if self._F is not None:
             out += self._F  # solver.py:243-244, before the projection
+        if self.turb_model is not None:  # solver.py:246-249
+            fxm, fym, fzm = self.turb_model.get_forcing(*state_spect[:3])
+            out[0] += fxm
+            out[1] += fym
+            out[2] += fzm
         self.project_state_spect(out)
         oper.dealiasing(out)
         return out
