@@ -1,16 +1,20 @@
 """Initial fields (fluidsim/base/init_fields.py:26-127, 435-443).
 
 Types: "in_script" (the state stays zero until the script calls
-``sim.state.init_statespect_from(...)``), "constant", and the
-solver-specific ones registered by subclasses (e.g. "noise").
-"from_file" (restart from a state file) is not implemented yet.
+``sim.state.init_statespect_from(...)``), "constant", "from_simul"
+(``sim.init_fields.get_state_from_simul(sim_in)`` copies the state of
+another simulation, changing the resolution on the device;
+fluidsim/base/init_fields.py:340-432), and the solver-specific ones
+registered by subclasses (e.g. "noise").  "from_file" (restart from a
+state file) is not available: no HDF5 / netCDF library in this build.
 """
 
+from .. import _lib
 from ..params import Parameters
 
 
 class InitFieldsBase:
-    available_types = ("in_script", "constant")
+    available_types = ("in_script", "constant", "from_simul")
 
     @classmethod
     def _complete_params_with_default(cls, params, info_solver=None):
@@ -49,6 +53,34 @@ class InitFieldsBase:
                 f"(available: {self.available_types})"
             )
         meth()
+
+    def init_from_simul(self):
+        """InitFieldsFromSimul.__call__ (init_fields.py:343-344)."""
+        self.get_state_from_simul = self._get_state_from_simul
+
+    def _get_state_from_simul(self, sim_in):
+        """init_fields.py:384-432: time and state of sim_in, resolution
+        changed with fill_field_fft_2d / fill_field_fft_3d on the device."""
+        sim = self.sim
+        if sim.output.name_solver != sim_in.output.name_solver:
+            raise NotImplementedError
+        sim.time_stepping.t = sim_in.time_stepping.t
+        src = sim_in.state.state_spect.data
+        dst = sim.state.state_spect
+        if tuple(src.shape) == tuple(dst.data.shape):
+            dst.data.copy_(src.to(dst.data.dtype))
+        else:
+            if src.dtype != dst.data.dtype:
+                src = src.to(dst.data.dtype)
+            src = src.contiguous()
+            dst.initialize(0.0)
+            si, so = tuple(src.shape[1:]), tuple(dst.data.shape[1:])
+            if len(si) == 2:
+                si, so = (1,) + si, (1,) + so
+            _lib.call("fsx_resize_fft", _lib.dptr(src), si[0], si[1], si[2],
+                      _lib.dptr(dst.data), so[0], so[1], so[2], src.shape[0],
+                      sim.oper.precision, _lib.stream_ptr())
+        sim.state.invalidate_phys()
 
     def _init_constant(self):
         """Every physical variable equals params.init_fields.constant.value:

@@ -49,6 +49,7 @@ class OutputBasePseudoSpectral:
             ) if s
         )
         self.name_run = name
+        self.name_solver = sim.info_solver.short_name
         self._has_to_save = bool(p.output.HAS_TO_SAVE)
         base = os.environ.get("FLUIDSIM_PATH", os.path.expanduser("~/Sim_data"))
         self.path_run = os.path.join(base, p.output.sub_directory, name)

@@ -1612,6 +1612,30 @@ int fsx_smagorinsky_div(fsx_plan p, const void* g_fft, void* out_fft, int accumu
   return 0;
 }
 
+int fsx_resize_fft(const void* in, int nz_in, int ny_in, int nkx_in, void* out,
+                   int nz_out, int ny_out, int nkx_out, int nfields, int precision,
+                   void* stream) {
+  if (!in || !out || nfields < 1 || nz_in < 1 || ny_in < 1 || nkx_in < 1 ||
+      nz_out < 1 || ny_out < 1 || nkx_out < 1)
+    return fail(FSX_ERR_ARG, "bad argument");
+  if (precision != 4 && precision != 8) return fail(FSX_ERR_ARG, "precision must be 4 or 8");
+  if ((nz_in == 1) != (nz_out == 1)) return fail(FSX_ERR_ARG, "2D and 3D arrays mixed");
+  hipStream_t s = (hipStream_t)stream;
+  const long long n = (long long)nz_out * ny_out * nkx_out * nfields;
+  if (precision == 4)
+    hipLaunchKernelGGL(k_resize<float>, dim3(grid_blocks(n)), dim3(256), 0, s,
+                       reinterpret_cast<const cpx<float>*>(in), nz_in, ny_in, nkx_in,
+                       reinterpret_cast<cpx<float>*>(out), nz_out, ny_out, nkx_out,
+                       nfields);
+  else
+    hipLaunchKernelGGL(k_resize<double>, dim3(grid_blocks(n)), dim3(256), 0, s,
+                       reinterpret_cast<const cpx<double>*>(in), nz_in, ny_in, nkx_in,
+                       reinterpret_cast<cpx<double>*>(out), nz_out, ny_out, nkx_out,
+                       nfields);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
 int fsx_timing_enable(fsx_plan p, int enable) {
   if (!p) return fail(FSX_ERR_ARG, "null plan");
   if (!p->tm) {

@@ -410,4 +410,40 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
   if (i < n) p[i] = v;
 }
 
+// Resolution change of nf K fields (fill_field_fft_2d,
+// fluidsim/base/init_fields.py:300-314; fill_field_fft_3d,
+// fluidsim/util/mini_oper_modif_resol.py:26-53): output mode (o0, o1, ox)
+// takes the input mode of the same signed wavenumbers when
+// -m/2 < k <= m/2 on each signed axis (m = min of the two sizes; index
+// <= m/2 is a non-negative wavenumber) and ox < min(nkx); the other output
+// modes are left unchanged.  nz = 1 for 2D.
+__device__ __forceinline__ int resize_src(int o, int n_out, int n_in) {
+  const int m = n_out < n_in ? n_out : n_in;
+  if (o <= m / 2) return o;
+  const int s = o - n_out;  // negative wavenumber index
+  return -s < m / 2 ? n_in + s : -1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_resize(const cpx<T>* in, int nz_in, int ny_in,
+                                                int nkx_in, cpx<T>* out, int nz_out,
+                                                int ny_out, int nkx_out, int nf) {
+  const long long ko = (long long)nz_out * ny_out * nkx_out;
+  const long long ki = (long long)nz_in * ny_in * nkx_in;
+  const int mkx = nkx_out < nkx_in ? nkx_out : nkx_in;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < ko * nf;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(i / ko);
+    const long long j = i - f * ko;
+    const int ox = (int)(j % nkx_out);
+    const long long r = j / nkx_out;
+    const int oy = (int)(r % ny_out), oz = (int)(r / ny_out);
+    if (ox >= mkx) continue;
+    const int sy = resize_src(oy, ny_out, ny_in);
+    const int sz = nz_out == 1 && nz_in == 1 ? 0 : resize_src(oz, nz_out, nz_in);
+    if (sy < 0 || sz < 0) continue;
+    out[i] = in[f * ki + ((long long)sz * ny_in + sy) * nkx_in + ox];
+  }
+}
+
 }  // namespace fsx

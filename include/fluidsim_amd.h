@@ -253,6 +253,17 @@ int fsx_smagorinsky_phys(fsx_plan p, void* S, double coef, void* stream);
 int fsx_smagorinsky_div(fsx_plan p, const void* g_fft, void* out_fft, int accumulate,
                         void* stream);
 
+/* Resolution change of nfields K arrays (no plan needed; nz = 1 in 2D):
+ * fill_field_fft_2d (fluidsim/base/init_fields.py:300-314) and
+ * fill_field_fft_3d (fluidsim/util/mini_oper_modif_resol.py:26-53).  Output
+ * modes whose signed wavenumbers satisfy -m/2 < k <= m/2 (m = the smaller
+ * size) on ky, kz and kx < min(nkx) take the input value; the others are
+ * left unchanged (zero the output first for a truncation / padding).
+ * precision 4 or 8 (complex64 / complex128). */
+int fsx_resize_fft(const void* in, int nz_in, int ny_in, int nkx_in, void* out,
+                   int nz_out, int ny_out, int nkx_out, int nfields, int precision,
+                   void* stream);
+
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its
  * stream.  fsx_timing_read (sync) returns, per launch kind
