@@ -83,6 +83,11 @@ SIGNATURES = {
     "fsx_strain_rate": [P, P, P, P],
     "fsx_smagorinsky_phys": [P, P, c_double, P],
     "fsx_smagorinsky_div": [P, P, P, c_int, P],
+    "fsx_if_combine": [
+        P, c_int, P, c_int, P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
+        ctypes.POINTER(c_double), ctypes.POINTER(c_int), ctypes.POINTER(c_double),
+        ctypes.POINTER(c_double), c_int, P,
+    ],
     "fsx_resize_fft": [P, c_int, c_int, c_int, P, c_int, c_int, c_int, c_int, c_int, P],
     "fsx_timing_enable": [P, c_int],
     "fsx_timing_read": [

@@ -107,20 +107,95 @@ class TimeSteppingPseudoSpectral:
         if p.USE_CFL and not self.sim.state.has_vars("ux") and not self.sim.state.has_vars("vx"):
             raise ValueError("params_ts.USE_CFL but no velocity.")
 
+    # schemes built from full tendencies and IF linear combinations
+    # (pseudo_spect.py:245-796)
+    _GENERIC_SCHEMES = (
+        "Euler", "Euler_phaseshift", "Euler_phaseshift_random", "RK2_trapezoid",
+        "RK2_phaseshift", "RK2_phaseshift_random", "RK2_phaseshift_exact",
+    )
+
     def _init_time_scheme(self):
         scheme = self.params.time_stepping.type_time_scheme
+        like = self.sim.state.state_spect
         if scheme == "RK4":
             self._scheme = 4
         elif scheme == "RK2":
             self._scheme = 2
+        elif scheme in self._GENERIC_SCHEMES:
+            self._scheme = 0
+            self._extra = [SetOfVariables(like=like, info=f"tmp{i}") for i in range(4)]
+            if scheme.endswith("_random"):
+                self._init_phaseshift_random()
         else:
             raise ValueError(f'Problem name time_scheme ("{scheme}")')
-        like = self.sim.state.state_spect
         self._state_spect_tmp = SetOfVariables(like=like, info="tmp")
         self._state_spect_tmp1 = SetOfVariables(like=like, info="tmp1")
         self._tendencies = SetOfVariables(like=like, info="tendencies_nonlin")
         self._visc = _lib.doubles(self.sim.visc_coefs())
         self._scal = (ctypes.c_double * 6)()
+
+    # ------------------------------------------------------------ phase shifts
+    def _shift_half(self):
+        """_get_phaseshift (pseudo_spect.py:281-300): a translation by half
+        a grid step along every axis, phase 0.5 (dx kx + dy ky + dz kz)."""
+        o = self.sim.oper
+        return (0.5 * o.deltax, 0.5 * o.deltay, 0.5 * getattr(o, "deltaz", 0.0))
+
+    def _random_pair(self):
+        """get_phases_random (operators2d.py:618-631, operators3d.py
+        :1128-1146): translations alpha dx and beta dx with alpha uniform
+        on [-1/2, 1/2) and |alpha - beta| = 1/2, per axis."""
+        o = self.sim.oper
+        deltas = (o.deltax, o.deltay) + ((o.deltaz,) if o.dim == 3 else ())
+        alphas = [self._uniform(-0.5, 0.5) for _ in deltas]
+        betas = [a + 0.5 if a < 0 else a - 0.5 for a in alphas]
+        sa = tuple(a * d for a, d in zip(alphas, deltas)) + (0.0,) * (3 - len(deltas))
+        sb = tuple(b * d for b, d in zip(betas, deltas)) + (0.0,) * (3 - len(deltas))
+        return sa, sb
+
+    def _init_phaseshift_random(self):
+        """pseudo_spect.py:302-327; the translations come from Python's
+        random module as in operators2d.py:13,618-631, so random.seed
+        reproduces a run."""
+        import random
+
+        self._uniform = random.uniform
+        p = self.params.time_stepping.phaseshift_random
+        if p.nb_steps_compute_new_pair is None:
+            p.nb_steps_compute_new_pair = 2 if p.nb_pairs == 1 else 4 * p.nb_pairs
+        self._index_phaseshift = 1
+        self._previous_index_pair = 0
+        self._previous_index_flip = 0
+        self._pairs_phaseshift = [self._random_pair() for _ in range(p.nb_pairs)]
+
+    def _get_phaseshift_random(self):
+        """pseudo_spect.py:329-371 (the pair rotation and renewal)."""
+        from random import randint
+
+        p = self.params.time_stepping.phaseshift_random
+        nb_pairs, nb_new = p.nb_pairs, p.nb_steps_compute_new_pair
+        if nb_pairs == 1 and nb_new == 1:
+            alpha, beta = self._pairs_phaseshift[0]
+        elif nb_pairs == 1 and nb_new == 2:
+            pair = self._pairs_phaseshift[0]
+            alpha, beta = pair if self._index_phaseshift == 1 else pair[::-1]
+        else:
+            index_pair = randint(0, nb_pairs - 1)
+            pair = self._pairs_phaseshift[index_pair]
+            index_flip = randint(0, 1)
+            if (index_pair == self._previous_index_pair
+                    and index_flip == self._previous_index_flip):
+                index_flip = 0 if index_flip else 1
+            self._previous_index_pair = index_pair
+            self._previous_index_flip = index_flip
+            alpha, beta = pair if index_flip else pair[::-1]
+        if self._index_phaseshift == nb_new:
+            self._index_phaseshift = 1
+            self._pairs_phaseshift.pop(0)
+            self._pairs_phaseshift.append(self._random_pair())
+        else:
+            self._index_phaseshift += 1
+        return alpha, beta
 
     @property
     def freq_lin(self):
@@ -266,10 +341,110 @@ class TimeSteppingPseudoSpectral:
                       _lib.dptr(S), _lib.dptr(tmp), _lib.dptr(t1), _lib.dptr(N),
                       proj, novz, self._visc, s)
 
+    # ------------------------------------------------------------ generic
+    def _combine(self, out, terms, shift=(0.0, 0.0, 0.0), check_nan=False):
+        """out = sum (a + b dt) exp(-e dt sigma) exp(i p shift.k) x over the
+        terms (x, a, b, e, p) (fsx_if_combine)."""
+        n = len(terms)
+        ins = (ctypes.c_void_p * n)(*[_lib.dptr(t[0].data) for t in terms])
+        a = _lib.doubles([t[1] for t in terms])
+        b = _lib.doubles([t[2] for t in terms])
+        e = _lib.doubles([t[3] for t in terms])
+        pw = (ctypes.c_int * n)(*[t[4] for t in terms])
+        _lib.call("fsx_if_combine", self.sim.oper.plan.handle, out.nvar,
+                  _lib.dptr(out.data), n, ins, a, b, e, pw, _lib.doubles(shift),
+                  self._visc, int(check_nan), self._stream())
+
+    def _tend(self, x, out):
+        self.sim.tendencies_nonlin(x, old=out)
+        return out
+
+    def _tend_shifted(self, x, shift, work, out):
+        """N(ps x) (the caller divides by ps in its combination)."""
+        self._combine(work, [(x, 1.0, 0.0, 0.0, 1)], shift)
+        return self._tend(work, out)
+
+    def _time_step_generic(self):
+        """Euler, RK2_trapezoid and the phase-shifted schemes
+        (pseudo_spect.py:245-796) from full tendencies and IF linear
+        combinations; the state S is updated in place."""
+        p = self.params.time_stepping
+        sim = self.sim
+        plan = sim.oper.plan.handle
+        s = self._stream()
+        self._push_dt()
+        if p.USE_CFL:
+            self.compute_time_increment_CLF()
+        maker = (sim.forcing.forcing_maker
+                 if getattr(sim, "is_forcing_enabled", False) else None)
+        if maker is not None and getattr(maker, "_pending", False):
+            _lib.call("fsx_forcing_normalize", plan, _lib.dptr(sim.state.state_spect.data), s)
+            maker._pending = False
+        S = sim.state.state_spect
+        T0, T1, T2, T3 = self._extra
+        W = self._state_spect_tmp1
+        scheme = p.type_time_scheme
+        h = self._shift_half()
+        if scheme == "Euler":
+            N0 = self._tend(S, T0)
+            self._combine(S, [(S, 1, 0, 1, 0), (N0, 0, 1, 1, 0)], check_nan=True)
+        elif scheme == "Euler_phaseshift":
+            N0 = self._tend(S, T0)
+            N0s = self._tend_shifted(S, h, W, T1)
+            self._combine(S, [(S, 1, 0, 1, 0), (N0, 0, 0.5, 1, 0), (N0s, 0, 0.5, 1, -1)],
+                          h, check_nan=True)
+        elif scheme == "Euler_phaseshift_random":
+            sa, sb = self._get_phaseshift_random()
+            Na = self._tend_shifted(S, sa, W, T0)
+            Nb = self._tend_shifted(S, sb, W, T1)
+            self._combine(T2, [(S, 1, 0, 1, 0), (Na, 0, 0.5, 1, -1)], sa)
+            self._combine(S, [(T2, 1, 0, 0, 0), (Nb, 0, 0.5, 1, -1)], sb, check_nan=True)
+        elif scheme == "RK2_trapezoid":
+            N0 = self._tend(S, T0)
+            self._combine(T1, [(S, 1, 0, 1, 0), (N0, 0, 1, 1, 0)])
+            N1 = self._tend(T1, T2)
+            self._combine(S, [(S, 1, 0, 1, 0), (N0, 0, 0.5, 1, 0), (N1, 0, 0.5, 0, 0)],
+                          check_nan=True)
+        elif scheme == "RK2_phaseshift":
+            N0 = self._tend(S, T0)
+            self._combine(T1, [(S, 1, 0, 1, 0), (N0, 0, 1, 1, 0)])
+            N1s = self._tend_shifted(T1, h, W, T2)
+            self._combine(S, [(S, 1, 0, 1, 0), (N0, 0, 0.5, 0.5, 0),
+                              (N1s, 0, 0.5, 0.5, -1)], h, check_nan=True)
+        elif scheme == "RK2_phaseshift_random":
+            sa, sb = self._get_phaseshift_random()
+            N0s = self._tend_shifted(S, sa, W, T0)
+            # N0 = N0s / ps_alpha
+            self._combine(T3, [(N0s, 1, 0, 0, -1)], sa)
+            self._combine(T1, [(S, 1, 0, 1, 0), (T3, 0, 1, 1, 0)])
+            N1s = self._tend_shifted(T1, sb, W, T2)
+            self._combine(S, [(S, 1, 0, 1, 0), (T3, 0, 0.5, 0.5, 0),
+                              (N1s, 0, 0.5, 0.5, -1)], sb, check_nan=True)
+        elif scheme == "RK2_phaseshift_exact":
+            N0 = self._tend(S, T0)
+            N0s = self._tend_shifted(S, h, W, T1)
+            # Nd0 = (N0 + N0s / ps) / 2 into T3
+            self._combine(T3, [(N0, 0.5, 0, 0, 0), (N0s, 0.5, 0, 0, -1)], h)
+            self._combine(T1, [(S, 1, 0, 1, 0), (T3, 0, 1, 1, 0)])
+            N1 = self._tend(T1, T0)
+            N1s = self._tend_shifted(T1, h, W, T2)
+            # S E + dt E2 (Nd0 / 2 + N1 / 4 + N1s / (4 ps))
+            self._combine(S, [(S, 1, 0, 1, 0), (T3, 0, 0.5, 0.5, 0), (N1, 0, 0.25, 0.5, 0),
+                              (N1s, 0, 0.25, 0.5, -1)], h, check_nan=True)
+        else:
+            raise ValueError(scheme)
+        # one_time_step_computation: projection (ns3d) and dealiasing
+        if hasattr(sim, "project_state_spect") and sim.oper.dim == 3:
+            sim.project_state_spect(S)
+        sim.oper.dealiasing(S)
+
     def one_time_step_computation(self):
         """pseudo_spect.py:236-243 / ns3d/time_stepping.py:8-20; projection
         and dealiasing of the new state are part of the last stage."""
-        self._time_step_RK()
+        if self._scheme == 0:
+            self._time_step_generic()
+        else:
+            self._time_step_RK()
         sim = self.sim
         sim.state.invalidate_phys()
         sc = self._read_scalars()

@@ -1612,6 +1612,64 @@ int fsx_smagorinsky_div(fsx_plan p, const void* g_fft, void* out_fft, int accumu
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int if_combine(fsx_plan_s* p, int nvar, void* out, int nin, void* const* ins,
+               const double* a, const double* b, const double* e, const int* pw,
+               const double* shift, const double* visc, int check_nan, hipStream_t s) {
+  IFCombine<T> c;
+  for (int j = 0; j < 4; ++j) {
+    const bool on = j < nin;
+    c.in[j] = on ? reinterpret_cast<const cpx<T>*>(ins[j]) : nullptr;
+    c.a[j] = on ? a[j] : 0.0;
+    c.b[j] = on ? b[j] : 0.0;
+    c.e[j] = on ? e[j] : 0.0;
+    c.p[j] = on ? pw[j] : 0;
+  }
+  c.nin = nin;
+  c.out = reinterpret_cast<cpx<T>*>(out);
+  c.ke = (long long)kelems(p);
+  c.nvar = nvar;
+  c.ax = shift[0];
+  c.ay = shift[1];
+  c.az = shift[2];
+  c.box = make_box(p, false);
+  c.visc.nu2 = visc[0];
+  c.visc.nu4 = visc[1];
+  c.visc.nu8 = visc[2];
+  c.visc.num4 = visc[3];
+  c.dt = p->d_scal + 4;
+  c.nanflag = check_nan ? reinterpret_cast<unsigned int*>(p->d_scal + 5) : nullptr;
+  hipLaunchKernelGGL(k_if_combine<T>, dim3(grid_blocks(c.ke)), dim3(256), 0, s, c);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsx_if_combine(fsx_plan p, int nvar, void* out, int nin, void* const* ins,
+                   const double* a, const double* b, const double* e,
+                   const int* phase_pow, const double* shift, const double* visc,
+                   int check_nan, void* stream) {
+  if (!p || !out || !ins || !a || !b || !e || !phase_pow || !shift || !visc)
+    return fail(FSX_ERR_ARG, "bad argument");
+  if (nin < 1 || nin > 4 || nvar < 1 || nvar > 4) return fail(FSX_ERR_ARG, "bad sizes");
+  for (int j = 0; j < nin; ++j) {
+    if (!ins[j]) return fail(FSX_ERR_ARG, "null input");
+    if (phase_pow[j] < -1 || phase_pow[j] > 1) return fail(FSX_ERR_ARG, "phase power must be -1, 0 or 1");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? if_combine<float>(p, nvar, out, nin, ins, a, b, e, phase_pow,
+                                          shift, visc, check_nan, s)
+                      : if_combine<double>(p, nvar, out, nin, ins, a, b, e, phase_pow,
+                                           shift, visc, check_nan, s);
+}
+
 int fsx_resize_fft(const void* in, int nz_in, int ny_in, int nkx_in, void* out,
                    int nz_out, int ny_out, int nkx_out, int nfields, int precision,
                    void* stream) {

@@ -410,6 +410,70 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
   if (i < n) p[i] = v;
 }
 
+// Integrating-factor linear combination over all modes of nvar fields,
+// the building block of the Euler / RK2 / phase-shifted schemes
+// (fluidsim/base/time_stepping/pseudo_spect.py:47-97, 245-796):
+//   out = sum_j (a_j + b_j dt) exp(-e_j dt sigma(k)) exp(i p_j phi(k)) in_j
+// with phi(k) = ax kx + ay ky + az kz (the phase shift of a translation by
+// (ax, ay, az)), dt the device time step.  out may alias one input.
+template <typename T>
+struct IFCombine {
+  const cpx<T>* in[4];
+  double a[4], b[4], e[4];
+  int p[4];
+  int nin;
+  cpx<T>* out;
+  long long ke;  // elements per field
+  int nvar;
+  double ax, ay, az;
+  Box box;
+  Visc visc;
+  const double* dt;
+  unsigned int* nanflag;  // set when a value of field 0 is NaN, or null
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_if_combine(IFCombine<T> c) {
+  const Box& bx = c.box;
+  const double dt = *c.dt;
+  const long long n = c.ke;
+  bool nan = false;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ikx = (int)(i % bx.nkx);
+    const long long r = i / bx.nkx;
+    const int iy = (int)(r % bx.ny), iz = (int)(r / bx.ny);
+    const int iky = ksigned(iy, bx.ny);
+    const int ikz = bx.dim == 3 ? ksigned(iz, bx.nz) : 0;
+    const double sig = freq_diss(bx, c.visc, ikx, iky, ikz);
+    const double phi = c.ax * ikx * bx.dkx + c.ay * iky * bx.dky + c.az * ikz * bx.dkz;
+    const double cph = cos(phi), sph = sin(phi);
+    double w[4];
+    cpx<double> ph[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= c.nin) break;
+      w[j] = (c.a[j] + c.b[j] * dt) * (c.e[j] != 0.0 ? exp(-c.e[j] * dt * sig) : 1.0);
+      ph[j] = c.p[j] == 0 ? mk<double>(1.0, 0.0)
+                          : mk<double>(cph, c.p[j] > 0 ? sph : -sph);
+    }
+    for (int v = 0; v < c.nvar; ++v) {
+      double sx = 0.0, sy = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j >= c.nin) break;
+        const cpx<T> x = c.in[j][v * n + i];
+        const double xr = x.x, xi = x.y;
+        sx += w[j] * (ph[j].x * xr - ph[j].y * xi);
+        sy += w[j] * (ph[j].x * xi + ph[j].y * xr);
+      }
+      c.out[v * n + i] = mk<T>(T(sx), T(sy));
+      if (v == 0 && (isnan(sx) || isnan(sy))) nan = true;
+    }
+  }
+  if (nan && c.nanflag) atomicOr(c.nanflag, 1u);
+}
+
 // Resolution change of nf K fields (fill_field_fft_2d,
 // fluidsim/base/init_fields.py:300-314; fill_field_fft_3d,
 // fluidsim/util/mini_oper_modif_resol.py:26-53): output mode (o0, o1, ox)

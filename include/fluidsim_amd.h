@@ -253,6 +253,20 @@ int fsx_smagorinsky_phys(fsx_plan p, void* S, double coef, void* stream);
 int fsx_smagorinsky_div(fsx_plan p, const void* g_fft, void* out_fft, int accumulate,
                         void* stream);
 
+/* Integrating-factor linear combination over all modes of nvar fields:
+ *   out = sum_{j<nin} (a_j + b_j dt) exp(-e_j dt sigma(k))
+ *                     exp(i phase_pow_j (shift . k)) in_j
+ * (dt: the device time step, sigma from visc = {nu2, nu4, nu8, nu_m4}),
+ * the step of the Euler, RK2-trapezoid and phase-shifted schemes
+ * (step_Euler, step_like_RK2, mean_with_phaseshift, mul, div_inplace;
+ * fluidsim/base/time_stepping/pseudo_spect.py:47-97, 245-796).  nin <= 4,
+ * phase_pow in {-1, 0, 1}; out may alias an input; check_nan sets the NaN
+ * flag read by fsx_get_scalars when field 0 of out holds a NaN. */
+int fsx_if_combine(fsx_plan p, int nvar, void* out, int nin, void* const* ins,
+                   const double* a, const double* b, const double* e,
+                   const int* phase_pow, const double* shift, const double* visc,
+                   int check_nan, void* stream);
+
 /* Resolution change of nfields K arrays (no plan needed; nz = 1 in 2D):
  * fill_field_fft_2d (fluidsim/base/init_fields.py:300-314) and
  * fill_field_fft_3d (fluidsim/util/mini_oper_modif_resol.py:26-53).  Output

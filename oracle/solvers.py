@@ -147,14 +147,107 @@ class _SolverBase:
         N1 = self.tendencies_nonlin(t12)
         S[...] = S * diss + dt * diss2 * N1
 
+    # --- Euler, RK2 trapezoid and phase-shifted schemes (pseudo_spect.py
+    # :245-796); random variants take their translations from
+    # self.phase_pairs (a callable returning two shift vectors per step)
+    def phaseshift(self, shift):
+        """exp(i shift . k) on the K grid (_get_phaseshift, 281-300)."""
+        o = self.oper
+        if self.state_spect.ndim == 3:
+            phase = shift[0] * o.KX + shift[1] * o.KY
+        else:
+            phase = shift[0] * o.Kx + shift[1] * o.Ky + shift[2] * o.Kz
+        return np.exp(1j * phase)
+
+    def half_shift(self):
+        o = self.oper
+        return (0.5 * o.deltax, 0.5 * o.deltay, 0.5 * getattr(o, "deltaz", 0.0))
+
+    def _time_step_Euler(self):
+        """pseudo_spect.py:245-272."""
+        dt = self.deltat
+        E, _ = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        N0 = self.tendencies_nonlin(S)
+        S[...] = (S + dt * N0) * E
+
+    def _time_step_Euler_phaseshift(self):
+        """pseudo_spect.py:374-419."""
+        dt = self.deltat
+        E, _ = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        ps = self.phaseshift(self.half_shift())
+        N0 = self.tendencies_nonlin(S)
+        Ns = self.tendencies_nonlin(ps * S) / ps
+        S[...] = (S + dt * 0.5 * (N0 + Ns)) * E
+
+    def _time_step_Euler_phaseshift_random(self):
+        """pseudo_spect.py:421-467."""
+        dt = self.deltat
+        E, _ = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        sa, sb = self.phase_pairs()
+        pa, pb = self.phaseshift(sa), self.phaseshift(sb)
+        Na = self.tendencies_nonlin(pa * S) / pa
+        Nb = self.tendencies_nonlin(pb * S) / pb
+        S[...] = (S + dt * 0.5 * (Na + Nb)) * E
+
+    def _time_step_RK2_trapezoid(self):
+        """pseudo_spect.py:519-569 (Heun)."""
+        dt = self.deltat
+        E, _ = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        N0 = self.tendencies_nonlin(S)
+        S1 = (S + dt * N0) * E
+        N1 = self.tendencies_nonlin(S1)
+        S[...] = (S + dt / 2 * N0) * E + dt / 2 * N1
+
+    def _time_step_RK2_phaseshift(self):
+        """pseudo_spect.py:571-639."""
+        dt = self.deltat
+        E, E2 = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        N0 = self.tendencies_nonlin(S)
+        S1 = (S + dt * N0) * E
+        ps = self.phaseshift(self.half_shift())
+        N1s = self.tendencies_nonlin(ps * S1)
+        Nd = 0.5 * (N0 + N1s / ps)
+        S[...] = S * E + dt * E2 * Nd
+
+    def _time_step_RK2_phaseshift_random(self):
+        """pseudo_spect.py:641-710."""
+        dt = self.deltat
+        E, E2 = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        sa, sb = self.phase_pairs()
+        pa, pb = self.phaseshift(sa), self.phaseshift(sb)
+        N0 = self.tendencies_nonlin(pa * S) / pa
+        S1 = (S + dt * N0) * E
+        N1s = self.tendencies_nonlin(pb * S1)
+        Nd = 0.5 * (N0 + N1s / pb)
+        S[...] = S * E + dt * E2 * Nd
+
+    def _time_step_RK2_phaseshift_exact(self):
+        """pseudo_spect.py:712-796."""
+        dt = self.deltat
+        E, E2 = self.exact_linear_coefs(dt)
+        S = self.state_spect
+        ps = self.phaseshift(self.half_shift())
+        N0 = self.tendencies_nonlin(S)
+        N0s = self.tendencies_nonlin(ps * S)
+        Nd0 = 0.5 * (N0 + N0s / ps)
+        S1 = (S + dt * Nd0) * E
+        N1 = self.tendencies_nonlin(S1)
+        N1s = self.tendencies_nonlin(ps * S1)
+        Nd = 0.5 * (Nd0 + 0.5 * (N1 + N1s / ps))
+        S[...] = S * E + dt * E2 * Nd
+
     def one_time_step_computation(self):
         """pseudo_spect.py:236-243 / ns3d/time_stepping.py:8-20."""
-        if self.type_time_scheme == "RK4":
-            self._time_step_RK4()
-        elif self.type_time_scheme == "RK2":
-            self._time_step_RK2()
-        else:
+        step = getattr(self, "_time_step_" + self.type_time_scheme, None)
+        if step is None:
             raise ValueError(self.type_time_scheme)
+        step()
         self.project_state_spect(self.state_spect)
         self.oper.dealiasing(self.state_spect)
         if np.isnan(np.sum(self.state_spect[0])):
