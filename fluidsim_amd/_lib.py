@@ -70,7 +70,6 @@ SIGNATURES = {
         P, c_int, P, P, P, P, c_double, ctypes.POINTER(c_double),
         ctypes.POINTER(c_double), P,
     ],
-    "fsx_spectrum": [P, P, P, P, c_int, c_int, ctypes.POINTER(c_double), P],
     "fsx_forcing_setup": [
         P, c_int, c_int, c_int, c_int, c_int, P, P, c_int, c_int,
     ],
@@ -89,6 +88,10 @@ SIGNATURES = {
         ctypes.POINTER(c_double), c_int, P,
     ],
     "fsx_resize_fft": [P, c_int, c_int, c_int, P, c_int, c_int, c_int, c_int, c_int, P],
+    "fsx_diag_layout": [P, c_int, c_int, ctypes.POINTER(ctypes.c_longlong)],
+    "fsx_diag": [P, c_int, c_int, P, c_int, ctypes.POINTER(c_double),
+                 ctypes.POINTER(c_double), P],
+    "fsx_ns2d_budget_fields": [P, P, c_double, P, P, P, P],
     "fsx_timing_enable": [P, c_int],
     "fsx_timing_read": [
         P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),

@@ -1,10 +1,12 @@
 """Output (fluidsim/base/output/base.py:60-521, subset).
 
-Periodic printing (print_stdout, periods_print.print_stdout) and spatial
-means (periods_save.spatial_means) computed with FP64 sums on the device;
-energy spectra on request.  Files (stdout.txt, spatial_means.txt,
-params.json) are written to path_run when params.output.HAS_TO_SAVE.
-Physical-field state files and plots are not implemented yet.
+Periodic printing (print_stdout, periods_print.print_stdout) and the
+specific outputs of fluidsim_amd.base.diagnostics, computed on the device:
+spatial_means, spectra and spect_energy_budg (periods_save.*), created at
+post_init, first saved when the state is initialised, then every period.
+Files (stdout.txt, params.json, spatial_means.txt, spectra / budget .npz) go
+to path_run when params.output.HAS_TO_SAVE.  Physical-field state files
+and plots are not implemented.
 """
 
 import json
@@ -29,8 +31,11 @@ class OutputBasePseudoSpectral:
         )
         params.output._set_child(
             "periods_save",
-            attribs={"phys_fields": 0, "spatial_means": 0, "spectra": 0},
+            attribs={"phys_fields": 0, "spatial_means": 0, "spectra": 0,
+                     "spect_energy_budg": 0},
         )
+        for tag in ("spatial_means", "spectra", "spect_energy_budg"):
+            params.output._set_child(tag, attribs={"HAS_TO_PLOT_SAVED": False})
         params.output._set_child("periods_print", attribs={"print_stdout": 1.0})
         params.output._set_child("periods_plot", attribs={"phys_fields": 0})
         params.output._set_child("phys_fields", attribs={"field_to_plot": ""})
@@ -54,11 +59,22 @@ class OutputBasePseudoSpectral:
         base = os.environ.get("FLUIDSIM_PATH", os.path.expanduser("~/Sim_data"))
         self.path_run = os.path.join(base, p.output.sub_directory, name)
         self._t_last = {}
-        self.spatial_means_results = []
         self._end_of_simul = False
+        self._specific = []
 
     # ----------------------------------------------------------------
+    # per-solver classes of the specific outputs (None: not available)
+    _cls_spatial_means = None
+    _cls_spectra = None
+    _cls_spect_energy_budg = None
+
     def post_init(self):
+        for name in ("spatial_means", "spectra", "spect_energy_budg"):
+            cls = getattr(self, "_cls_" + name)
+            if cls is not None:
+                obj = cls(self)
+                setattr(self, name, obj)
+                self._specific.append(obj)
         if self._has_to_save:
             os.makedirs(self.path_run, exist_ok=True)
             with open(os.path.join(self.path_run, "params.json"), "w") as f:
@@ -82,9 +98,13 @@ class OutputBasePseudoSpectral:
         if self.params.time_stepping.USE_CFL:
             self.sim.time_stepping.compute_time_increment_CLF()
         self._do_outputs(force=True)
+        for obj in self._specific:
+            obj._init_files()
 
     def one_time_step(self):
         self._do_outputs(force=False)
+        for obj in self._specific:
+            obj._online_save()
 
     def _due(self, key, period, t, force):
         if not period:
@@ -103,21 +123,13 @@ class OutputBasePseudoSpectral:
             self.print_stdout(
                 f"it = {ts.it:6d} ; t = {t:12.6g} ; deltat  = {ts.deltat:10.5g}"
             )
-        if self._due("spatial_means", p.periods_save.spatial_means, t, force):
-            self.save_spatial_means()
 
-    def save_spatial_means(self):
-        means = self.compute_spatial_means()
-        means["t"] = self.sim.time_stepping.t
-        self.spatial_means_results.append(means)
-        if self._has_to_save:
-            with open(os.path.join(self.path_run, "spatial_means.txt"), "a") as f:
-                f.write(json.dumps(means) + "\n")
-        return means
+    def compute_spatial_means(self):
+        """One spatial-means record (device sums)."""
+        return self.spatial_means.compute()
 
     def load_spatial_means(self):
-        keys = self.spatial_means_results[0].keys() if self.spatial_means_results else []
-        return {k: np.array([m[k] for m in self.spatial_means_results]) for k in keys}
+        return self.spatial_means.load()
 
     def end_of_simul(self, total_time=None):
         self._end_of_simul = True
@@ -131,14 +143,3 @@ class OutputBasePseudoSpectral:
                   float(N2), _lib.doubles(self.sim.visc_coefs()), out,
                   _lib.stream_ptr())
         return list(out)
-
-    def compute_spectrum(self, fields, nbins=None):
-        """Shell spectrum sum_{round(|k|/dk)=j} w |f|^2/2 / dk (FP64)."""
-        if nbins is None:
-            nbins = self.oper.nx // 2 + 1
-        out = (_lib.ctypes.c_double * nbins)()
-        f = [_lib.dptr(a) for a in fields] + [None] * (3 - len(fields))
-        _lib.call("fsx_spectrum", self.oper.plan.handle, f[0], f[1], f[2],
-                  len(fields), nbins, out, _lib.stream_ptr())
-        khE = self.oper.deltakx * np.arange(nbins)
-        return khE, np.array(out)

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include "fsx_rows.h"
 #include "fsx_forcing.h"
 #include "fsx_turb.h"
+#include "fsx_diag.h"
 
 using namespace fsx;
 
@@ -81,8 +83,6 @@ struct fsx_plan_s {
   size_t w1_elems, w2_elems;  // capacity in complex elements
   int nf_max;
   double* d_scal;  // [0..3] maxima bits, [4] dt, [5] nan flag, [8..15] sums
-  double* d_spec;
-  int spec_cap;
   // coarse forcing box (fsx_forcing_*)
   bool force_on, force_pending;
   void* d_fc;
@@ -96,6 +96,11 @@ struct fsx_plan_s {
   int fkey_kind, fkey_var;
   double fcoef, frate;
   int froot;
+  // diagnostics: per-block records and the output (fsx_diag)
+  double* d_dpart;
+  size_t dpart_cap;
+  double* d_dout;
+  size_t dout_cap;
 };
 
 namespace {
@@ -1121,12 +1126,13 @@ int fsx_plan_destroy(fsx_plan p) {
   if (p->w2) (void)hipFree(p->w2);
   if (p->d_mask) (void)hipFree(p->d_mask);
   if (p->d_scal) (void)hipFree(p->d_scal);
-  if (p->d_spec) (void)hipFree(p->d_spec);
   if (p->d_fc) (void)hipFree(p->d_fc);
   if (p->d_fmodes) (void)hipFree(p->d_fmodes);
   if (p->d_fw) (void)hipFree(p->d_fw);
   if (p->d_ff0) (void)hipFree(p->d_ff0);
   if (p->d_ff1) (void)hipFree(p->d_ff1);
+  if (p->d_dpart) (void)hipFree(p->d_dpart);
+  if (p->d_dout) (void)hipFree(p->d_dout);
   delete p;
   return 0;
 }
@@ -1379,37 +1385,6 @@ int fsx_sums(fsx_plan p, int kind, const void* f0, const void* f1,
   return p->prec == 4
              ? do_sums<float>(p, kind, f0, f1, f2, f3, N2, visc, host_out6, s)
              : do_sums<double>(p, kind, f0, f1, f2, f3, N2, visc, host_out6, s);
-}
-
-int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
-                 int nf, int nbins, double* host_out, void* stream) {
-  if (!p || !f0 || nf < 1 || nf > 3 || nbins < 1 || !host_out)
-    return fail(FSX_ERR_ARG, "bad argument");
-  hipStream_t s = (hipStream_t)stream;
-  if (p->spec_cap < nbins) {
-    if (p->d_spec) (void)hipFree(p->d_spec);
-    p->d_spec = nullptr;
-    FSX_HIP(hipMalloc((void**)&p->d_spec, nbins * sizeof(double)));
-    p->spec_cap = nbins;
-  }
-  FSX_HIP(hipMemsetAsync(p->d_spec, 0, nbins * sizeof(double), s));
-  const size_t n = kelems(p);
-  const Box box = make_box(p, false);
-  if (!f1) f1 = f0;
-  if (!f2) f2 = f0;
-  if (p->prec == 4)
-    hipLaunchKernelGGL(k_spectrum<float>, dim3(grid_for(n)), dim3(256), 0, s,
-                       (const cpx<float>*)f0, (const cpx<float>*)f1,
-                       (const cpx<float>*)f2, nf, box, nbins, p->d_spec);
-  else
-    hipLaunchKernelGGL(k_spectrum<double>, dim3(grid_for(n)), dim3(256), 0, s,
-                       (const cpx<double>*)f0, (const cpx<double>*)f1,
-                       (const cpx<double>*)f2, nf, box, nbins, p->d_spec);
-  FSX_HIP(hipGetLastError());
-  FSX_HIP(hipMemcpyAsync(host_out, p->d_spec, nbins * sizeof(double),
-                         hipMemcpyDeviceToHost, s));
-  FSX_HIP(hipStreamSynchronize(s));
-  return 0;
 }
 
 int fsx_forcing_setup(fsx_plan p, int nyc, int nzc, int nkxc, int nvar,
@@ -1732,6 +1707,242 @@ int fsx_timing_read(fsx_plan p, double* ms, double* bytes, long long* counts,
     }
   }
   return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// LDS of one block record is kept within 64 KiB (two blocks per CU)
+constexpr int kDiagLdsDoubles = 8192;
+
+int diag_geom(const fsx_plan_s* p, int kind, int flags, DiagGeom& g, DiagOut& o,
+              long long& total) {
+  if (kind < 0 || kind >= DIAG_NKINDS) return fail(FSX_ERR_ARG, "unknown diagnostics kind");
+  const bool means = kind >= DIAG_NS2D_MEANS;
+  const bool kind2d = kind == DIAG_NS2D_SPECTRA || kind == DIAG_NS2D_BUDGET ||
+                      kind == DIAG_NS2D_MEANS;
+  if (kind2d != (p->dim == 2))
+    return fail(FSX_ERR_ARG, "diagnostics kind does not match the plan dimension");
+  if (flags & ~31) return fail(FSX_ERR_ARG, "unknown histogram flags");
+  if (means && flags) return fail(FSX_ERR_ARG, "the spatial-means kinds take no histograms");
+  if (p->dim == 2 && (flags & (DIAG_KZ | DIAG_KZKH)))
+    return fail(FSX_ERR_ARG, "kz histograms need a 3D plan");
+  g = DiagGeom{};
+  g.dim = p->dim;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.nz = p->nz;
+  g.nkx = p->nkx;
+  g.dkx = p->dkx;
+  g.dky = p->dky;
+  g.dkz = p->dim == 3 ? p->dkz : 1.0;
+  g.nkxs = p->nx / 2 + 1;
+  g.nkys = p->ny / 2 + 1;
+  g.nkzs = p->dim == 3 ? p->nz / 2 + 1 : 1;
+  if (p->dim == 3) {
+    // k_spectra3d and kh_spectra of the 3D operators
+    g.dk3 = std::max(p->dkx, std::max(p->dky, p->dkz));
+    const double kmax = std::min(p->dkx * g.nkxs, std::min(p->dky * g.nkys, p->dkz * g.nkzs));
+    g.nk3 = std::max(2, (int)(kmax / g.dk3));
+    g.dkh = std::max(p->dkx, p->dky);
+    const double khmax = std::min(p->dkx * g.nkxs, p->dky * g.nkys);
+    g.nkh = std::max(2, (int)(khmax / g.dkh));
+  } else {
+    // khE = kxE of the 2D operators
+    g.dk3 = p->dkx;
+    g.nk3 = g.nkxs;
+    g.dkh = p->dkx;
+    g.nkh = 1;
+  }
+  g.flags = flags;
+  const int Q = diag_nq(kind);
+  o.Q = Q;
+  long long off = 0;
+  o.o_kx = off;
+  if (flags & DIAG_KX) off += (long long)Q * g.nkxs;
+  o.o_ky = off;
+  if (flags & DIAG_KY) off += (long long)Q * g.nkys;
+  o.o_kz = off;
+  if (flags & DIAG_KZ) off += (long long)Q * g.nkzs;
+  o.o_k3 = off;
+  if (flags & DIAG_SHELL) off += (long long)Q * g.nk3;
+  o.o_kzkh = off;
+  if (flags & DIAG_KZKH) off += (long long)Q * g.nkzs * g.nkh;
+  o.o_sum = off;
+  total = off + Q;
+  // blocks: one kz plane and a run of ky rows each, about 512 blocks
+  int nchunk = (512 + p->nz - 1) / p->nz;
+  nchunk = std::max(1, std::min(nchunk, std::max(1, p->ny / 4)));
+  g.rows = (p->ny + nchunk - 1) / nchunk;
+  g.nchunk = (p->ny + g.rows - 1) / g.rows;
+  return 0;
+}
+
+int diag_record(DiagGeom& g, int q0, int nq) {
+  g.q0 = q0;
+  g.nq = nq;
+  g.off_kx = 0;
+  g.off_ky = g.off_kx + ((g.flags & DIAG_KX) ? nq * g.nkxs : 0);
+  g.off_k3 = g.off_ky + ((g.flags & DIAG_KY) ? nq * g.nkys : 0);
+  g.off_kh = g.off_k3 + ((g.flags & DIAG_SHELL) ? nq * g.nk3 : 0);
+  g.off_sum = g.off_kh + ((g.flags & DIAG_KZKH) ? nq * g.nkh : 0);
+  g.rec = g.off_sum + nq;
+  return g.rec;
+}
+
+int diag_per_quantity(const DiagGeom& g) {
+  return ((g.flags & DIAG_KX) ? g.nkxs : 0) + ((g.flags & DIAG_KY) ? g.nkys : 0) +
+         ((g.flags & DIAG_SHELL) ? g.nk3 : 0) + ((g.flags & DIAG_KZKH) ? g.nkh : 0) + 1;
+}
+
+template <typename T, int KIND>
+void diag_launch(const DiagIn& in, const Box& box, const DiagGeom& g, double* part,
+                 hipStream_t s) {
+  const unsigned nb = (unsigned)(g.nz * g.nchunk);
+  hipLaunchKernelGGL((k_diag<T, KIND>), dim3(nb), dim3(256), g.rec * sizeof(double), s, in,
+                     box, g, part);
+}
+
+template <typename T>
+int diag_dispatch(int kind, const DiagIn& in, const Box& box, const DiagGeom& g,
+                  double* part, hipStream_t s) {
+  switch (kind) {
+    case DIAG_NS2D_SPECTRA: diag_launch<T, DIAG_NS2D_SPECTRA>(in, box, g, part, s); break;
+    case DIAG_NS2D_BUDGET: diag_launch<T, DIAG_NS2D_BUDGET>(in, box, g, part, s); break;
+    case DIAG_NS3D_SPECTRA: diag_launch<T, DIAG_NS3D_SPECTRA>(in, box, g, part, s); break;
+    case DIAG_STRAT_SPECTRA: diag_launch<T, DIAG_STRAT_SPECTRA>(in, box, g, part, s); break;
+    case DIAG_NS3D_BUDGET: diag_launch<T, DIAG_NS3D_BUDGET>(in, box, g, part, s); break;
+    case DIAG_STRAT_BUDGET: diag_launch<T, DIAG_STRAT_BUDGET>(in, box, g, part, s); break;
+    case DIAG_NS2D_MEANS: diag_launch<T, DIAG_NS2D_MEANS>(in, box, g, part, s); break;
+    case DIAG_NS3D_MEANS: diag_launch<T, DIAG_NS3D_MEANS>(in, box, g, part, s); break;
+    case DIAG_STRAT_MEANS: diag_launch<T, DIAG_STRAT_MEANS>(in, box, g, part, s); break;
+    default: return fail(FSX_ERR_ARG, "unknown diagnostics kind");
+  }
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int grow(double*& buf, size_t& cap, size_t n) {
+  if (cap >= n) return 0;
+  if (buf) (void)hipFree(buf);
+  buf = nullptr;
+  cap = 0;
+  FSX_HIP(hipMalloc((void**)&buf, n * sizeof(double)));
+  cap = n;
+  return 0;
+}
+
+template <typename T>
+int ns2d_budget(fsx_plan_s* p, const void* rot, double beta, void* wk, void* wx, void* out,
+                hipStream_t s) {
+  const long long ke = (long long)kelems(p), xe = (long long)xelems(p);
+  hipLaunchKernelGGL(k_ns2d_budget_grads<T>, dim3(grid_blocks(ke)), dim3(256), 0, s,
+                     reinterpret_cast<const cpx<T>*>(rot), reinterpret_cast<cpx<T>*>(wk), ke,
+                     p->nkx, p->ny, p->dkx, p->dky);
+  FSX_HIP(hipGetLastError());
+  int rc = do_ifft<T>(p, wk, wx, 8, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ns2d_budget_prod<T>, dim3(grid_blocks(xe)), dim3(256), 0, s,
+                     reinterpret_cast<T*>(wx), xe, beta);
+  FSX_HIP(hipGetLastError());
+  return do_fft<T>(p, wx, out, 3, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsx_diag_layout(fsx_plan p, int kind, int flags, long long* out8) {
+  if (!p || !out8) return fail(FSX_ERR_ARG, "bad argument");
+  DiagGeom g;
+  DiagOut o;
+  long long total;
+  int rc = diag_geom(p, kind, flags, g, o, total);
+  if (rc) return rc;
+  const int nin_req = kind == DIAG_NS2D_MEANS   ? 1
+                      : kind == DIAG_NS3D_MEANS ? 3
+                      : kind == DIAG_STRAT_MEANS ? 4
+                                                 : diag_nin(kind);
+  out8[0] = o.Q;
+  out8[1] = g.nkxs;
+  out8[2] = g.nkys;
+  out8[3] = g.nkzs;
+  out8[4] = g.nk3;
+  out8[5] = g.nkh;
+  out8[6] = total;
+  out8[7] = nin_req;
+  return 0;
+}
+
+int fsx_diag(fsx_plan p, int kind, int flags, const void* const* fields, int nfields,
+             const double* params, double* host_out, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !fields || !params || !host_out) return fail(FSX_ERR_ARG, "bad argument");
+  DiagGeom g;
+  DiagOut o;
+  long long total;
+  int rc = diag_geom(p, kind, flags, g, o, total);
+  if (rc) return rc;
+  const int nin = diag_nin(kind);
+  const int nreq = kind == DIAG_NS2D_MEANS   ? 1
+                   : kind == DIAG_NS3D_MEANS ? 3
+                   : kind == DIAG_STRAT_MEANS ? 4
+                                              : nin;
+  if (nfields < nreq || nfields > nin) return fail(FSX_ERR_ARG, "wrong number of fields");
+  DiagIn in{};
+  for (int f = 0; f < nin; ++f) {
+    in.f[f] = f < nfields ? fields[f] : nullptr;
+    if (f < nreq && !in.f[f]) return fail(FSX_ERR_ARG, "null field");
+  }
+  if (nfields > nreq) {
+    // optional forcing fields come all together
+    for (int f = nreq; f < nin; ++f)
+      if (!in.f[f]) return fail(FSX_ERR_ARG, "incomplete forcing fields");
+  }
+  in.inv_N2 = params[0];
+  in.visc.nu2 = params[1];
+  in.visc.nu4 = params[2];
+  in.visc.nu8 = params[3];
+  in.visc.num4 = params[4];
+  const int per_q = diag_per_quantity(g);
+  if (per_q > kDiagLdsDoubles) return fail(FSX_ERR_ARG, "grid too large for the LDS histograms");
+  const int Q = o.Q;
+  const int qstep = std::max(1, std::min(Q, kDiagLdsDoubles / per_q));
+  const size_t nb = (size_t)g.nz * g.nchunk;
+  DiagGeom gmax = g;
+  rc = grow(p->d_dpart, p->dpart_cap, nb * (size_t)diag_record(gmax, 0, qstep));
+  if (rc) return rc;
+  rc = grow(p->d_dout, p->dout_cap, (size_t)total);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Box box = make_box(p, false);
+  for (int q0 = 0; q0 < Q; q0 += qstep) {
+    DiagGeom gq = g;
+    diag_record(gq, q0, std::min(qstep, Q - q0));
+    rc = p->prec == 4 ? diag_dispatch<float>(kind, in, box, gq, p->d_dpart, s)
+                      : diag_dispatch<double>(kind, in, box, gq, p->d_dpart, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_diag_reduce, dim3(grid_blocks(total)), dim3(256), 0, s, p->d_dpart,
+                       gq, o, p->d_dout);
+    FSX_HIP(hipGetLastError());
+  }
+  FSX_HIP(hipMemcpyAsync(host_out, p->d_dout, (size_t)total * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  FSX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsx_ns2d_budget_fields(fsx_plan p, const void* rot_fft, double beta, void* work_k,
+                           void* work_x, void* out_fft, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !rot_fft || !work_k || !work_x || !out_fft) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 2) return fail(FSX_ERR_ARG, "the ns2d budget needs a 2D plan");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = p->prec == 4 ? ns2d_budget<float>(p, rot_fft, beta, work_k, work_x, out_fft, s)
+                        : ns2d_budget<double>(p, rot_fft, beta, work_k, work_x, out_fft, s);
+  if (rc) return rc;
+  return fsx_dealias(p, out_fft, 3, 0, stream);
 }
 
 }  // extern "C"

@@ -358,37 +358,6 @@ __global__ __launch_bounds__(256) void k_sums(const cpx<T>* f0,
   }
 }
 
-// Shell spectrum E(k_j) = sum over shell j of w |f|^2 / 2 / dk with
-// j = round(|k| / dk), dk = dkx (sum over nf fields).
-template <typename T>
-__global__ __launch_bounds__(256) void k_spectrum(const cpx<T>* f0,
-                                                  const cpx<T>* f1,
-                                                  const cpx<T>* f2, int nf,
-                                                  Box box, int nbins,
-                                                  double* out) {
-  const size_t n = (size_t)box.nz * box.ny * box.nkx;
-  const bool even = (box.nx & 1) == 0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int ikx = (int)(i % box.nkx);
-    const size_t rest = i / box.nkx;
-    const int iy = (int)(rest % box.ny), iz = (int)(rest / box.ny);
-    const double kx = ikx * box.dkx, ky = ksigned(iy, box.ny) * box.dky;
-    const double kz = box.dim == 3 ? ksigned(iz, box.nz) * box.dkz : 0.0;
-    const double kk = sqrt(kx * kx + ky * ky + kz * kz);
-    const int j = (int)rint(kk / box.dkx);
-    if (j >= nbins) continue;
-    const double w = (ikx == 0 || (even && ikx == box.nx / 2)) ? 1.0 : 2.0;
-    double e = 0;
-    const cpx<T>* fs[3] = {f0, f1, f2};
-    for (int q = 0; q < nf; ++q) {
-      const cpx<T> a = fs[q][i];
-      e += 0.5 * ((double)a.x * a.x + (double)a.y * a.y);
-    }
-    atomicAdd(out + j, w * e / box.dkx);
-  }
-}
-
 // CFL update (fluidsim/base/time_stepping/base.py:320-366): tmp =
 // sum_i max|u_i| / dx_i, dt_CFL = CFL / tmp, new = min(dt_CFL, dt_max),
 // kept unless the relative change exceeds 2 %.

@@ -10,9 +10,9 @@
  * Streams are hipStream_t passed as void* (NULL = default stream).
  * Every function returns 0 on success or a negative FSX_ERR_* code;
  * fsx_last_error() copies the message of the last failure of the thread.
- * Calls are stream-ordered; only fsx_get_scalars, fsx_sums, fsx_spectrum and
- * fsx_set_dt synchronise the stream.  A plan must not be used from two host
- * threads at once.
+ * Calls are stream-ordered; only the entry points marked (sync), such as
+ * fsx_get_scalars, fsx_sums, fsx_diag and fsx_set_dt, synchronise the
+ * stream.  A plan must not be used from two host threads at once.
  *
  * Each entry point names the reference interface it replaces
  * (paths relative to the fluidsim source tree).
@@ -180,11 +180,6 @@ int fsx_sums(fsx_plan p, int kind, const void* f0, const void* f1,
              const void* f2, const void* f3, double N2, const double* visc,
              double* host_out6, void* stream);
 
-/* Shell spectrum of nf fields: out[j] = sum over round(|k|/dkx) = j of
- * w |f|^2 / 2 / dkx, j < nbins (sync). */
-int fsx_spectrum(fsx_plan p, const void* f0, const void* f1, const void* f2,
-                 int nf, int nbins, double* host_out, void* stream);
-
 /* ---- forcing on a coarse spectral box ----------------------------------
  * fluidsim computes its forcing on a coarse grid and adds the full-size
  * copy to the tendencies at every stage
@@ -277,6 +272,56 @@ int fsx_if_combine(fsx_plan p, int nvar, void* out, int nin, void* const* ins,
 int fsx_resize_fft(const void* in, int nz_in, int ny_in, int nkx_in, void* out,
                    int nz_out, int ny_out, int nkx_out, int nfields, int precision,
                    void* stream);
+
+/* ---- diagnostics: spectra, spectral energy budgets, spatial means ------
+ * One pass over the K arrays with FP64 accumulation.  kind selects the
+ * per-mode quantities (Q of them) and the input fields:
+ *   0 ns2d spectra     (rot)                      Q=1  E
+ *     fluidsim/solvers/ns2d/output/spectra.py:16-28
+ *   1 ns2d budget      (rot, Frot, Fx, Fy)        Q=2  transferE, transferZ
+ *     fluidsim/solvers/ns2d/output/spect_energy_budget.py:23-80
+ *   2 ns3d spectra     (vx, vy, vz)               Q=5  vx vy vz Khr Khd
+ *     fluidsim/solvers/ns3d/output/spectra.py:15-99
+ *   3 strat spectra    (vx, vy, vz, b)            Q=6  vx vy vz A Khr Khd
+ *     fluidsim/solvers/ns3d/strat/output/spectra.py:18-112
+ *   4 ns3d budget      (v, P fft(v x omega))      Q=5  transfer_Kh transfer_Kz
+ *                      transfer_Khr diss_Kh diss_Kz
+ *     fluidsim/solvers/ns3d/output/spect_energy_budget.py:106-190
+ *   5 strat budget     (v, f, b, b tendency)      Q=8  + diss_A conv_K2A transfer_A
+ *     fluidsim/solvers/ns3d/strat/output/spect_energy_budget.py:60-118
+ *   6 ns2d means       (rot [, Frot])             Q=10 E Z epsK epsK_hypo epsZ
+ *                      epsZ_hypo PK1 PK2' PZ1 PZ2'
+ *     fluidsim/solvers/ns2d/output/spatial_means.py:23-68
+ *   7 ns3d means       (v [, f])                  Q=9  Ex Ey Ez epsK epsK_hypo
+ *                      epsK4 epsK8 PK1 PK2'
+ *     fluidsim/solvers/ns3d/output/spatial_means.py:23-77
+ *   8 strat means      (v, b [, f, fb])           Q=18 A Kz Khr Khd Khs As epsK
+ *                      epsK_hypo epsA epsA_hypo epsK4 epsA4 epsK8 epsA8 PK1 PK2'
+ *                      PA1' PA2'
+ *     fluidsim/solvers/ns3d/strat/output/spatial_means.py:27-106
+ * (primed terms lack the factor dt/2 and, for PA, 1/N^2).  flags (spectra
+ * and budgets only): 1 kx, 2 ky, 4 kz 1D spectra (bins |i_k|, / dk_i),
+ * 8 shell spectrum (3D: dk = max dk_i, nk = max(2, int(min_i(dk_i (n_i/2+1))
+ * / dk)); 2D: dk = dkx, nk = nx/2+1; floor bin and linear share with the
+ * next bin, / dk), 16 kz-kh spectrum (kh bins as the 3D shell with dkx,
+ * dky, / (dkz dkh)), replacing fluidfft's compute_1dspectra,
+ * compute_3dspectrum, compute_spectrum_kzkh, spectra1D_from_fft and
+ * spectrum2D_from_fft.  Each mode carries its Parseval weight.
+ * fsx_diag_layout: out8 = Q, nkx, nky, nkz, nk (shell), nkh, total doubles,
+ * required fields.  fsx_diag (sync) writes host_out[total] =
+ * [kx Q*nkx][ky Q*nky][kz Q*nkz][shell Q*nk][kzkh Q*nkz*nkh][sums Q] (absent
+ * parts take no room).  params = {1/N^2, nu_2, nu_4, nu_8, nu_m4}. */
+int fsx_diag_layout(fsx_plan p, int kind, int flags, long long* out8);
+int fsx_diag(fsx_plan p, int kind, int flags, const void* const* fields, int nfields,
+             const double* params, double* host_out, void* stream);
+
+/* ns2d budget nonlinear terms: out_fft (3 K fields) = dealiased fft of
+ * Frot = -ux d_x rot - uy (d_y rot + beta), Fx = -ux d_x ux - uy d_y ux,
+ * Fy = -ux d_x uy - uy d_y uy (u from rot), with caller workspace work_k
+ * (8 K fields) and work_x (8 X fields)
+ * (fluidsim/solvers/ns2d/output/spect_energy_budget.py:27-57). */
+int fsx_ns2d_budget_fields(fsx_plan p, const void* rot_fft, double beta, void* work_k,
+                           void* work_x, void* out_fft, void* stream);
 
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its

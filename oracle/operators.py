@@ -322,6 +322,18 @@ class Oper3D(_OperBase):
         vy_fft[...] = cp * tmp
         vz_fft[...] = 0.0
 
+    def urudfft_from_vxvyfft(self, vx_fft, vy_fft):
+        """Toroidal / poloidal split of the horizontal velocity
+        (operators3d.py:713-730): ud = k_h (k_h . v_h) / k_h^2 with
+        k_h^2 = 0 replaced by 1e-14, ur = v_h - ud."""
+        kh2 = self.Kx**2 + self.Ky**2
+        kh2[kh2 == 0] = 1e-14
+        inv = 1 / kh2
+        kdotu = self.Kx * vx_fft + self.Ky * vy_fft
+        udx = kdotu * self.Kx * inv
+        udy = kdotu * self.Ky * inv
+        return vx_fft - udx, vy_fft - udy, udx, udy
+
     def vtfft_from_vecfft(self, vx_fft, vy_fft, vz_fft):
         """operators3d.py:939-955."""
         _, _, cp, sp = self._angles()

@@ -1,0 +1,294 @@
+"""GPU parity of the on-device diagnostics (fsx_diag) against the FP64
+oracle (oracle/diagnostics.py): spectra, spectral energy budgets and
+spatial means of ns2d, ns3d and ns3d.strat
+(fluidsim/solvers/*/output/{spectra,spect_energy_budget,spatial_means}.py).
+
+The oracle reads the product's own state (copied to the host), so both
+sides bin the same values.  Bars: spectra and means of FP64 and FP32
+states within 1e-11 of the largest value of each array (both accumulate
+in FP64); budgets, whose nonlinear terms come from the device transforms,
+within 1e-10 (FP64).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import diagnostics as D
+from oracle.operators import Oper2D, Oper3D
+from oracle.solvers import NS2D, NS3D
+from fluidsim_amd.base.diagnostics import parse_spatial_means
+from fluidsim_amd.synthetic import ns2d_noise_rot_fft, ns3d_spectrum_vv_fft
+
+pytestmark = pytest.mark.gpu
+
+
+def close(got, ref, tol, name=""):
+    got, ref = np.asarray(got), np.asarray(ref)
+    assert got.shape == ref.shape, (name, got.shape, ref.shape)
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(got - ref).max() / scale
+    assert err <= tol, (name, err)
+    return err
+
+
+def _common(params, precision="float64", L=2 * np.pi):
+    params.oper.Lx = params.oper.Ly = L
+    if hasattr(params.oper, "Lz"):
+        params.oper.Lz = L
+    params.oper.precision = precision
+    params.time_stepping.USE_CFL = False
+    params.time_stepping.deltat0 = 0.01
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+
+
+def ns2d_sim(n, precision="float64", **extra):
+    from fluidsim_amd.solvers.ns2d.solver import Simul
+
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = n
+    params.nu_2 = 1e-3
+    params.nu_m4 = 1e-2
+    _common(params, precision)
+    for k, v in extra.items():
+        setattr(params, k, v)
+    sim = Simul(params)
+    sim.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(n, n, seed=31, kmax=12))
+    return sim
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_ns2d_spectra_budget_means(precision):
+    n = 64
+    print(f"case ns2d diagnostics n={n} {precision}", flush=True)
+    sim = ns2d_sim(n, precision)
+    rot = sim.state.get_var_numpy("rot_fft").astype(np.complex128)
+    oper = Oper2D(n, n)
+    d1, d2 = sim.output.spectra.compute()
+    r1, r2 = D.ns2d_spectra(oper, rot)
+    for k in r1:
+        close(d1[k], r1[k], 1e-11, k)
+    close(d2["spectrum2D_E"], r2["spectrum2D_E"], 1e-11, "spectrum2D_E")
+    orc = NS2D(oper, nu_2=1e-3, nu_m4=1e-2)
+    f_d, f_d_hypo = orc.compute_freq_diss()
+    m = sim.output.spatial_means.compute()
+    rm = D.ns2d_means(oper, rot, f_d, f_d_hypo)
+    for k in rm:
+        assert m[k] == pytest.approx(rm[k], rel=1e-11, abs=1e-13 * abs(rm["E"])), k
+    if precision == "float64":
+        T = sim.output.spect_energy_budg.compute()
+        rT = D.ns2d_budget(oper, rot)
+        for k in rT:
+            print(k, close(T[k], rT[k], 1e-10, k))
+
+
+def test_ns2d_beta_budget_and_forced_means():
+    from fluidsim_amd.solvers.ns2d.solver import Simul
+
+    n = 64
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = n
+    params.nu_2 = 1e-3
+    params.beta = 2.0
+    _common(params)
+    params.forcing.enable = True
+    params.forcing.type = "tcrandom"
+    params.forcing.nkmin_forcing = 3
+    params.forcing.nkmax_forcing = 5
+    params.forcing.forcing_rate = 0.4
+    params.time_stepping.it_end = 1
+    params.time_stepping.USE_T_END = False
+    np.random.seed(5)
+    sim = Simul(params)
+    sim.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(n, n, seed=32, kmax=12))
+    rot0 = sim.state.get_var_numpy("rot_fft")
+    sim.time_stepping.start()
+    # forcing normalised at the start of the step on rot0 with this dt
+    dt = sim.time_stepping.deltat
+    F = sim.forcing.get_forcing().to_numpy()[0]
+    oper = Oper2D(n, n)
+    orc = NS2D(oper, nu_2=1e-3)
+    f_d, f_d_hypo = orc.compute_freq_diss()
+    rm0 = D.ns2d_means(oper, rot0, f_d, f_d_hypo, F, dt)
+    assert rm0["PZ1"] + rm0["PZ2"] == pytest.approx(0.4, rel=1e-10)
+    rot = sim.state.get_var_numpy("rot_fft")
+    m = sim.output.spatial_means.compute()
+    rm = D.ns2d_means(oper, rot, f_d, f_d_hypo, F, dt)
+    for k in rm:
+        assert m[k] == pytest.approx(rm[k], rel=1e-11, abs=1e-13 * abs(rm["E"])), k
+    T = sim.output.spect_energy_budg.compute()
+    rT = D.ns2d_budget(oper, rot, beta=2.0)
+    for k in rT:
+        close(T[k], rT[k], 1e-10, k)
+
+
+def ns3d_sim(shape, L=(2 * np.pi,) * 3, precision="float64", strat=False, nu_4=0.0,
+             nu_8=0.0, kzkh_periodicity=1):
+    if strat:
+        from fluidsim_amd.solvers.ns3d.strat.solver import Simul
+    else:
+        from fluidsim_amd.solvers.ns3d.solver import Simul
+    nx, ny, nz = shape
+    params = Simul.create_default_params()
+    params.oper.nx, params.oper.ny, params.oper.nz = nx, ny, nz
+    _common(params, precision)
+    params.oper.Lx, params.oper.Ly, params.oper.Lz = L
+    params.nu_2 = 1e-2
+    params.nu_4 = nu_4
+    params.nu_8 = nu_8
+    params.output.spectra.kzkh_periodicity = kzkh_periodicity
+    if strat:
+        params.N = 2.5
+    sim = Simul(params)
+    rng = np.random.default_rng(33)
+    oper = Oper3D(nx, ny, nz, *L)
+    fields = []
+    for _ in range(4 if strat else 3):
+        x = rng.standard_normal(oper.shapeX)
+        a = oper.fft(x)
+        a[np.sqrt(oper.K2) > 0.4 * min(nx * oper.deltakx, ny * oper.deltaky,
+                                       nz * oper.deltakz)] = 0
+        fields.append(a)
+    S = np.array(fields)
+    oper.project_perpk3d(*S[:3])
+    oper.dealiasing(S)
+    keys = ["vx_fft", "vy_fft", "vz_fft"] + (["b_fft"] if strat else [])
+    sim.state.init_statespect_from(**dict(zip(keys, S)))
+    return sim, oper
+
+
+@pytest.mark.parametrize("case", [
+    dict(shape=(32, 32, 32)),
+    dict(shape=(32, 16, 8), L=(2 * np.pi, 3.0, 5.0), nu_4=1e-5, nu_8=1e-12),
+    dict(shape=(32, 32, 32), precision="float32"),
+])
+def test_ns3d_spectra_budget_means(case):
+    print(f"case ns3d diagnostics {case}", flush=True)
+    sim, oper = ns3d_sim(**case)
+    S = sim.state.state_spect.to_numpy().astype(np.complex128)
+    d1, d3, dk = sim.output.spectra.compute()
+    r1, r3, rk = D.ns3d_spectra(oper, *S)
+    for k in r1:
+        close(d1[k], r1[k], 1e-11, k)
+    for k in r3:
+        close(d3[k], r3[k], 1e-11, k)
+    for k in rk:
+        close(dk[k], rk[k], 1e-11, k)
+    nu_4, nu_8 = case.get("nu_4", 0.0), case.get("nu_8", 0.0)
+    orc = NS3D(oper, nu_2=1e-2, nu_4=nu_4, nu_8=nu_8)
+    f_d, f_d_hypo = orc.compute_freq_diss()
+    m = sim.output.spatial_means.compute()
+    rm = D.ns3d_means(oper, *S, f_d, f_d_hypo, nu_4, nu_8)
+    assert set(m) >= set(rm)
+    for k in rm:
+        assert m[k] == pytest.approx(rm[k], rel=1e-11, abs=1e-13 * abs(rm["E"])), k
+    if case.get("precision", "float64") == "float64":
+        T = sim.output.spect_energy_budg.compute()
+        rT = D.ns3d_budget(oper, *S, f_d)
+        assert set(T) == set(rT)
+        for k in rT:
+            close(T[k], rT[k], 1e-10, k)
+
+
+def test_strat_spectra_budget_means():
+    sim, oper = ns3d_sim((32, 32, 16), L=(2 * np.pi, 2 * np.pi, np.pi), strat=True,
+                         nu_4=1e-5)
+    print("case strat diagnostics 32x32x16", flush=True)
+    S = sim.state.state_spect.to_numpy()
+    N = 2.5
+    d1, d3, dk = sim.output.spectra.compute()
+    r1, r3, rk = D.ns3d_spectra(oper, *S[:3], b=S[3], N=N)
+    assert set(d1) == set(r1) and set(d3) == set(r3) and set(dk) == set(rk)
+    for d, r in ((d1, r1), (d3, r3), (dk, rk)):
+        for k in r:
+            close(d[k], r[k], 1e-11, k)
+    f_d, f_d_hypo = NS3D(oper, nu_2=1e-2, nu_4=1e-5).compute_freq_diss()
+    m = sim.output.spatial_means.compute()
+    rm = D.strat_means(oper, *S, N, f_d, f_d_hypo, nu_4=1e-5)
+    for k in rm:
+        assert m[k] == pytest.approx(rm[k], rel=1e-11, abs=1e-13 * abs(rm["E"])), k
+    T = sim.output.spect_energy_budg.compute()
+    rT = D.ns3d_budget(oper, *S[:3], f_d, b=S[3], N=N)
+    assert set(T) == set(rT)
+    for k in rT:
+        close(T[k], rT[k], 1e-10, k)
+
+
+def test_strat_forced_means():
+    from fluidsim_amd.solvers.ns3d.strat.solver import Simul
+
+    n = 32
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = params.oper.nz = n
+    _common(params)
+    params.N = 3.0
+    params.nu_2 = 1e-2
+    params.forcing.enable = True
+    params.forcing.type = "tcrandom"
+    params.forcing.key_forced = "b_fft"
+    params.forcing.nkmin_forcing = 2
+    params.forcing.nkmax_forcing = 4
+    params.time_stepping.it_end = 1
+    params.time_stepping.USE_T_END = False
+    np.random.seed(7)
+    sim = Simul(params)
+    vv = ns3d_spectrum_vv_fft(n, seed=34)
+    b = ns3d_spectrum_vv_fft(n, seed=35)[0]
+    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2], b_fft=b)
+    sim.time_stepping.start()
+    S = sim.state.state_spect.to_numpy()
+    F = sim.forcing.get_forcing().to_numpy()
+    dt = sim.time_stepping.deltat
+    oper = Oper3D(n, n, n)
+    f_d, f_d_hypo = NS3D(oper, nu_2=1e-2).compute_freq_diss()
+    m = sim.output.spatial_means.compute()
+    rm = D.strat_means(oper, *S, 3.0, f_d, f_d_hypo, f=F[:3], fb=F[3], deltat=dt)
+    for k in rm:
+        assert m[k] == pytest.approx(rm[k], rel=1e-11, abs=1e-13 * abs(rm["E"])), k
+    assert m["PA1"] != 0.0 and m["PK1"] == 0.0
+
+
+def test_saved_outputs(tmp_path, monkeypatch):
+    """Periodic saving: spatial_means.txt in the reference text format,
+    spectra / budget .npz with one record per save (t = 0 included)."""
+    from fluidsim_amd.solvers.ns3d.solver import Simul
+
+    monkeypatch.setenv("FLUIDSIM_PATH", str(tmp_path))
+    n = 16
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = params.oper.nz = n
+    _common(params)
+    params.output.HAS_TO_SAVE = True
+    params.nu_2 = 1e-2
+    params.time_stepping.USE_T_END = False
+    params.time_stepping.it_end = 4
+    params.output.periods_save.spatial_means = 1e-9
+    params.output.periods_save.spectra = 0.02
+    params.output.periods_save.spect_energy_budg = 0.02
+    params.output.spectra.kzkh_periodicity = 2
+    sim = Simul(params)
+    vv = ns3d_spectrum_vv_fft(n, seed=36)
+    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+    sim.time_stepping.start()
+    path = sim.output.path_run
+    means = parse_spatial_means(path + "/spatial_means.txt")
+    mem = sim.output.spatial_means.load()
+    # outputs run at the start of each step (t before the increment)
+    assert len(means["t"]) == 4
+    np.testing.assert_allclose(means["E"], mem["E"], rtol=1e-5)
+    np.testing.assert_allclose(means["t"], [0, 0.01, 0.02, 0.03], atol=1e-12)
+    with np.load(path + "/spectra1d.npz") as f:
+        assert f["spectra_E_kx"].shape == (2, n // 2 + 1)
+        np.testing.assert_allclose(f["times"], [0, 0.02], atol=1e-12)
+    with np.load(path + "/spectra_kzkh.npz") as f:
+        assert f["K"].shape[0] == 1  # first of every 2 saves
+    with np.load(path + "/spect_energy_budg.npz") as f:
+        assert f["transfer_Kh"].shape[0] == 2
+        # sum of the kz-kh transfer of K vanishes (spect_energy_budget.py:188-189)
+        TK = f["transfer_Kh"][-1] + f["transfer_Kz"][-1]
+        assert abs(TK.sum()) < 1e-12 * np.abs(f["transfer_Kh"][-1]).sum()
+    # dE/dt = -eps between the saved means (RK4, 4 steps)
+    E, eps, t = mem["E"], mem["epsK_tot"], mem["t"]
+    rate = np.diff(E) / np.diff(t)
+    assert np.allclose(rate, -0.5 * (eps[1:] + eps[:-1]), rtol=2e-3)

@@ -56,11 +56,16 @@ def test_strat_tendency_vs_oracle_and_energy():
     T = sum(np.real(tend[i].conj() * S0[i]) for i in range(3))
     T = T + np.real(tend[3].conj() * S0[3]) / N**2
     assert abs(oper.sum_wavenumbers(T) / oper.sum_wavenumbers(np.abs(T))) < 1e-13
-    EK, EA = sim.output.compute_energies()
-    oEK, oEA = NS3DStrat(oper, N=N), None
-    oEK.state_spect[:] = S0
-    rEK, rEA = oEK.compute_energies()
-    assert EK == pytest.approx(rEK, rel=1e-12) and EA == pytest.approx(rEA, rel=1e-12)
+    # compute_energies returns (E_A, E_Kz, E_Khr, E_Khd)
+    # (fluidsim/solvers/ns3d/strat/output/__init__.py:55-72, 86-88)
+    EA, EKz, EKhr, EKhd = sim.output.compute_energies()
+    orc_e = NS3DStrat(oper, N=N)
+    orc_e.state_spect[:] = S0
+    rEK, rEA = orc_e.compute_energies()
+    assert EKz + EKhr + EKhd == pytest.approx(rEK, rel=1e-12)
+    assert EA == pytest.approx(rEA, rel=1e-12)
+    assert EKz == pytest.approx(oper.sum_wavenumbers(np.abs(S0[2]) ** 2 / 2), rel=1e-12)
+    assert sim.output.compute_energy() == pytest.approx(rEK + rEA, rel=1e-12)
 
 
 @pytest.mark.parametrize("extra", [{}, {"f": 1.0}])
