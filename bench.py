@@ -33,7 +33,7 @@ if ROOT not in sys.path:
 METRIC = "RK4 steps/s + %HBM roofline; closure-LES vs DNS speedup at equal error"
 PEAK_HBM_GBPS = 8000.0
 KINDS = ["z-pass inverse", "y-pass inverse", "x row pass", "y-pass forward",
-         "z-pass forward", "RK stage", "other", "unused"]
+         "z-pass forward", "RK stage", "other", "diagnostics pass"]
 
 
 def b_alg_ns3d(n, prec):
@@ -120,6 +120,44 @@ def measured_copy_bandwidth(torch, nbytes=1 << 30, reps=5):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+def time_diagnostics(sim, plan, torch, reps=3):
+    """One save of each on-device output at the bench state (spectra with
+    kz-kh, spectral energy budget, spatial means): wall time per call and
+    the fsx_diag binning pass (live HIP events, engine timing slot 7),
+    whose algorithmic bytes are the input fields read once plus the block
+    records written."""
+    from fluidsim_amd import _lib
+
+    out = sim.output
+    jobs = (("spectra", lambda: out.spectra.compute(kzkh=True)),
+            ("spect_energy_budg", out.spect_energy_budg.compute),
+            ("spatial_means", out.spatial_means.compute))
+    ms8 = (_lib.ctypes.c_double * 8)()
+    by8 = (_lib.ctypes.c_double * 8)()
+    cn8 = (_lib.ctypes.c_longlong * 8)()
+    res = {}
+    for name, fn in jobs:
+        fn()
+        _lib.call("fsx_timing_enable", plan, 1)
+        _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+        _lib.call("fsx_timing_enable", plan, 0)
+        gbps = by8[7] / (ms8[7] * 1e-3) / 1e9 if ms8[7] else None
+        res[name] = {
+            "ms_per_call": wall * 1e3,
+            "diag_pass_ms_per_call": ms8[7] / reps,
+            "diag_pass_alg_GBps": gbps,
+            "diag_pass_frac": gbps / PEAK_HBM_GBPS if gbps else None,
+        }
+    return res
+
+
 def read_traffic(kernel_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -194,6 +232,7 @@ def main():
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
     steps_per_s = args.steps / elapsed
+    diagnostics = time_diagnostics(sim, plan, torch)
     kernels = {}
     for k in range(7):
         if cn8[k]:
@@ -249,6 +288,7 @@ def main():
             "frac_of_copy": balg * steps_per_s / 1e9 / copy_gbps,
         },
         "kernels": kernels,
+        "diagnostics": diagnostics,
     }
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)

@@ -430,6 +430,9 @@ struct DiagOut {
   int Q;
 };
 
+// One wave per output value: the lanes sum strided subsets of the
+// contributing block records, then a fixed shuffle tree adds the lanes
+// (deterministic for a given grid).
 __global__ __launch_bounds__(256) void k_diag_reduce(const double* partial, DiagGeom g,
                                                      DiagOut o, double* out) {
   const int nq = g.nq;
@@ -441,68 +444,63 @@ __global__ __launch_bounds__(256) void k_diag_reduce(const double* partial, Diag
   const long long n_sum = nq;
   const long long total = n_kx + n_ky + n_kz + n_k3 + n_kh + n_sum;
   const int nb = g.nz * g.nchunk;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long t = blockIdx.x * (long long)(blockDim.x >> 6) + (threadIdx.x >> 6);
+       t < total; t += nwaves) {
     long long e = t;
+    // contributing records: every block, or (jz >= 0) the blocks of the
+    // planes iz = jz and iz = nz - jz
+    int src = 0, jz = -1;
+    long long dst = 0;
+    double scale = 1.0;
     if (e < n_kx) {
       const int s = (int)(e / g.nkxs), j = (int)(e % g.nkxs);
-      double acc = 0;
-      for (int b = 0; b < nb; ++b) acc += partial[(size_t)b * g.rec + g.off_kx + e];
-      out[o.o_kx + (long long)(g.q0 + s) * g.nkxs + j] = acc / g.dkx;
-      continue;
-    }
-    e -= n_kx;
-    if (e < n_ky) {
+      src = g.off_kx + (int)e;
+      dst = o.o_kx + (long long)(g.q0 + s) * g.nkxs + j;
+      scale = 1.0 / g.dkx;
+    } else if ((e -= n_kx) < n_ky) {
       const int s = (int)(e / g.nkys), j = (int)(e % g.nkys);
-      double acc = 0;
-      for (int b = 0; b < nb; ++b) acc += partial[(size_t)b * g.rec + g.off_ky + e];
-      out[o.o_ky + (long long)(g.q0 + s) * g.nkys + j] = acc / g.dky;
-      continue;
-    }
-    e -= n_ky;
-    if (e < n_kz) {
-      const int s = (int)(e / g.nkzs), j = (int)(e % g.nkzs);
-      // planes iz = j and iz = nz - j carry |kz| = j dkz
-      double acc = 0;
-      const int iz2 = (j == 0 || 2 * j == g.nz) ? -1 : g.nz - j;
-      for (int c = 0; c < g.nchunk; ++c) {
-        acc += partial[(size_t)(j * g.nchunk + c) * g.rec + g.off_sum + s];
-        if (iz2 >= 0) acc += partial[(size_t)(iz2 * g.nchunk + c) * g.rec + g.off_sum + s];
-      }
-      out[o.o_kz + (long long)(g.q0 + s) * g.nkzs + j] = acc / g.dkz;
-      continue;
-    }
-    e -= n_kz;
-    if (e < n_k3) {
+      src = g.off_ky + (int)e;
+      dst = o.o_ky + (long long)(g.q0 + s) * g.nkys + j;
+      scale = 1.0 / g.dky;
+    } else if ((e -= n_ky) < n_kz) {
+      const int s = (int)(e / g.nkzs);
+      jz = (int)(e % g.nkzs);
+      src = g.off_sum + s;
+      dst = o.o_kz + (long long)(g.q0 + s) * g.nkzs + jz;
+      scale = 1.0 / g.dkz;
+    } else if ((e -= n_kz) < n_k3) {
       const int s = (int)(e / g.nk3), j = (int)(e % g.nk3);
-      double acc = 0;
-      for (int b = 0; b < nb; ++b) acc += partial[(size_t)b * g.rec + g.off_k3 + e];
-      out[o.o_k3 + (long long)(g.q0 + s) * g.nk3 + j] = acc / g.dk3;
-      continue;
-    }
-    e -= n_k3;
-    if (e < n_kh) {
+      src = g.off_k3 + (int)e;
+      dst = o.o_k3 + (long long)(g.q0 + s) * g.nk3 + j;
+      scale = 1.0 / g.dk3;
+    } else if ((e -= n_k3) < n_kh) {
       const long long per = (long long)g.nkzs * g.nkh;
       const int s = (int)(e / per);
-      const int jz = (int)((e % per) / g.nkh), jh = (int)(e % g.nkh);
+      jz = (int)((e % per) / g.nkh);
+      const int jh = (int)(e % g.nkh);
+      src = g.off_kh + s * g.nkh + jh;
+      dst = o.o_kzkh + (long long)(g.q0 + s) * per + (long long)jz * g.nkh + jh;
+      scale = 1.0 / (g.dkz * g.dkh);
+    } else {
+      e -= n_kh;
+      src = g.off_sum + (int)e;
+      dst = o.o_sum + g.q0 + e;
+    }
+    double acc = 0.0;
+    if (jz < 0) {
+      for (int b = lane; b < nb; b += 64) acc += partial[(size_t)b * g.rec + src];
+    } else {
       const int iz2 = (jz == 0 || 2 * jz == g.nz) ? -1 : g.nz - jz;
-      double acc = 0;
-      const int src = g.off_kh + s * g.nkh + jh;
-      for (int c = 0; c < g.nchunk; ++c) {
-        acc += partial[(size_t)(jz * g.nchunk + c) * g.rec + src];
-        if (iz2 >= 0) acc += partial[(size_t)(iz2 * g.nchunk + c) * g.rec + src];
+      const int m = (iz2 >= 0 ? 2 : 1) * g.nchunk;
+      for (int k = lane; k < m; k += 64) {
+        const int b = k < g.nchunk ? jz * g.nchunk + k : iz2 * g.nchunk + (k - g.nchunk);
+        acc += partial[(size_t)b * g.rec + src];
       }
-      out[o.o_kzkh + (long long)(g.q0 + s) * per + (long long)jz * g.nkh + jh] =
-          acc / (g.dkz * g.dkh);
-      continue;
     }
-    e -= n_kh;
-    {
-      const int s = (int)e;
-      double acc = 0;
-      for (int b = 0; b < nb; ++b) acc += partial[(size_t)b * g.rec + g.off_sum + s];
-      out[o.o_sum + g.q0 + s] = acc;
-    }
+    acc = wave_sum(acc);
+    if (lane == 0) out[dst] = acc * scale;
   }
 }
 

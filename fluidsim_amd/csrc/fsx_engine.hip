@@ -107,7 +107,7 @@ namespace {
 
 // kinds of launches reported by fsx_timing_read
 enum { K_ZINV = 0, K_YINV = 1, K_XPASS = 2, K_YFWD = 3, K_ZFWD = 4, K_RK = 5,
-       K_OTHER = 6 };
+       K_OTHER = 6, K_DIAG = 7 };
 
 thread_local fsx_plan_s* g_plan = nullptr;  // plan of the running API call
 thread_local int g_kind = K_OTHER;
@@ -1797,30 +1797,37 @@ int diag_per_quantity(const DiagGeom& g) {
 }
 
 template <typename T, int KIND>
-void diag_launch(const DiagIn& in, const Box& box, const DiagGeom& g, double* part,
-                 hipStream_t s) {
+int diag_launch(const DiagIn& in, const Box& box, const DiagGeom& g, double* part,
+                hipStream_t s) {
   const unsigned nb = (unsigned)(g.nz * g.nchunk);
+  // compulsory bytes: every input field read once, the block records written
+  int nin = 0;
+  for (int f = 0; f < 8; ++f) nin += in.f[f] != nullptr;
+  note(K_DIAG, (double)nin * g.nz * g.ny * g.nkx * sizeof(cpx<T>) +
+                   (double)nb * g.rec * sizeof(double));
+  int rc = tm_pre(s);
+  if (rc) return rc;
   hipLaunchKernelGGL((k_diag<T, KIND>), dim3(nb), dim3(256), g.rec * sizeof(double), s, in,
                      box, g, part);
+  FSX_HIP(hipGetLastError());
+  return tm_post(s);
 }
 
 template <typename T>
 int diag_dispatch(int kind, const DiagIn& in, const Box& box, const DiagGeom& g,
                   double* part, hipStream_t s) {
   switch (kind) {
-    case DIAG_NS2D_SPECTRA: diag_launch<T, DIAG_NS2D_SPECTRA>(in, box, g, part, s); break;
-    case DIAG_NS2D_BUDGET: diag_launch<T, DIAG_NS2D_BUDGET>(in, box, g, part, s); break;
-    case DIAG_NS3D_SPECTRA: diag_launch<T, DIAG_NS3D_SPECTRA>(in, box, g, part, s); break;
-    case DIAG_STRAT_SPECTRA: diag_launch<T, DIAG_STRAT_SPECTRA>(in, box, g, part, s); break;
-    case DIAG_NS3D_BUDGET: diag_launch<T, DIAG_NS3D_BUDGET>(in, box, g, part, s); break;
-    case DIAG_STRAT_BUDGET: diag_launch<T, DIAG_STRAT_BUDGET>(in, box, g, part, s); break;
-    case DIAG_NS2D_MEANS: diag_launch<T, DIAG_NS2D_MEANS>(in, box, g, part, s); break;
-    case DIAG_NS3D_MEANS: diag_launch<T, DIAG_NS3D_MEANS>(in, box, g, part, s); break;
-    case DIAG_STRAT_MEANS: diag_launch<T, DIAG_STRAT_MEANS>(in, box, g, part, s); break;
+    case DIAG_NS2D_SPECTRA: return diag_launch<T, DIAG_NS2D_SPECTRA>(in, box, g, part, s);
+    case DIAG_NS2D_BUDGET: return diag_launch<T, DIAG_NS2D_BUDGET>(in, box, g, part, s);
+    case DIAG_NS3D_SPECTRA: return diag_launch<T, DIAG_NS3D_SPECTRA>(in, box, g, part, s);
+    case DIAG_STRAT_SPECTRA: return diag_launch<T, DIAG_STRAT_SPECTRA>(in, box, g, part, s);
+    case DIAG_NS3D_BUDGET: return diag_launch<T, DIAG_NS3D_BUDGET>(in, box, g, part, s);
+    case DIAG_STRAT_BUDGET: return diag_launch<T, DIAG_STRAT_BUDGET>(in, box, g, part, s);
+    case DIAG_NS2D_MEANS: return diag_launch<T, DIAG_NS2D_MEANS>(in, box, g, part, s);
+    case DIAG_NS3D_MEANS: return diag_launch<T, DIAG_NS3D_MEANS>(in, box, g, part, s);
+    case DIAG_STRAT_MEANS: return diag_launch<T, DIAG_STRAT_MEANS>(in, box, g, part, s);
     default: return fail(FSX_ERR_ARG, "unknown diagnostics kind");
   }
-  FSX_HIP(hipGetLastError());
-  return 0;
 }
 
 int grow(double*& buf, size_t& cap, size_t n) {
@@ -1923,7 +1930,9 @@ int fsx_diag(fsx_plan p, int kind, int flags, const void* const* fields, int nfi
     rc = p->prec == 4 ? diag_dispatch<float>(kind, in, box, gq, p->d_dpart, s)
                       : diag_dispatch<double>(kind, in, box, gq, p->d_dpart, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_diag_reduce, dim3(grid_blocks(total)), dim3(256), 0, s, p->d_dpart,
+    // one wave per output value
+    const long long nred = std::min<long long>((total + 3) / 4, 65536);
+    hipLaunchKernelGGL(k_diag_reduce, dim3((unsigned)nred), dim3(256), 0, s, p->d_dpart,
                        gq, o, p->d_dout);
     FSX_HIP(hipGetLastError());
   }

@@ -180,8 +180,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < rem ? x * (q + 1) + local : rem * (q + 1) + (x - rem) * q + local;
 }
 
+// Minimum waves per SIMD asked of the strided passes (0: compiler's
+// choice).  Only the multi-transform z-passes use more registers than
+// this allows by default.
+#ifndef FSX_STRIDED_WAVES
+#define FSX_STRIDED_WAVES 0
+#endif
+#if FSX_STRIDED_WAVES > 0
+#define FSX_STRIDED_ATTR __attribute__((amdgpu_waves_per_eu(FSX_STRIDED_WAVES)))
+#else
+#define FSX_STRIDED_ATTR
+#endif
+
 template <typename T, int DIR, int EPT, int kT, class Op>
-__global__ __launch_bounds__(kT) void k_strided(Op op) {
+__global__ __launch_bounds__(kT) FSX_STRIDED_ATTR void k_strided(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const StridedDesc<T>& d = op.d;
   cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);

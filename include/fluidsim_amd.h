@@ -327,7 +327,8 @@ int fsx_ns2d_budget_fields(fsx_plan p, const void* rot_fft, double beta, void* w
  * When enabled, HIP events are recorded around every pass launch on its
  * stream.  fsx_timing_read (sync) returns, per launch kind
  * (0 z-pass inverse, 1 y-pass inverse, 2 x row pass, 3 y-pass forward,
- * 4 z-pass forward, 5 Runge-Kutta stage, 6 other, 7 unused), the summed
+ * 4 z-pass forward, 5 Runge-Kutta stage, 6 other, 7 diagnostics pass of
+ * fsx_diag), the summed
  * duration in ms, the summed algorithmic bytes (compulsory HBM reads +
  * writes of each launch) and the launch count; reset zeroes them. */
 int fsx_timing_enable(fsx_plan p, int enable);

@@ -32,8 +32,13 @@ IF-RK4 / RK2                N=0 exact decay, 4th/2nd order self-convergence,
                             linear ODE closed form per stage
 CFL                         scripted maxima, hysteresis rule
 energy budget               dE/dt = -eps to O(dt^4)
-spectra binning             parity unpinned (fluidfft binning not in tree); sum
-                            rule sum(E_k)*dk = E pins the normalisation only
+spectra binning             restated from fluidfft's published algorithm; full-
+                            grid brute force, sum rules, single-mode share
+                            (parity vs fluidfft itself unpinned: not in tree)
+budgets / spatial means     rate of change of the spectra under the oracle's
+                            own step (order 1), injection = forcing rate
+Euler / RK2 / phase shifts  exact IF decay, order of convergence, alias-free
+                            reduction of the shifted tendency
 ==========================  ===================================================
 """
 
