@@ -101,6 +101,12 @@ struct fsx_plan_s {
   size_t dpart_cap;
   double* d_dout;
   size_t dout_cap;
+  // batch of nb independent 2D runs (fsx_plan_create_batch): per-run
+  // {dt, nu_2, nu_4, nu_8, nu_m4} and the lines-per-tile cap of the batched
+  // y-passes (0: the launcher's choice)
+  int nb;
+  double* d_bpar;
+  int blines;
 };
 
 namespace {
@@ -213,7 +219,7 @@ int set_lds(K kern, size_t lds) {
 
 template <typename T, int DIR, class Op>
 int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw,
-                   hipStream_t s, bool kio = false) {
+                   hipStream_t s, bool kio = false, int lines_cap = 0) {
   constexpr int E = ept<T>() / Op::kEptDiv >= 8 ? ept<T>() / Op::kEptDiv : 8;
   const int tile = E * kThr;
   if (N > tile) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
@@ -222,8 +228,13 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   int cap = 1;
   while (cap < nlines) cap <<= 1;
   if (L > cap) L = cap;
-  // enough workgroups to fill the chip when the problem is small
-  while (L > 1 && (long long)nfb * nouter * ((nlines + L - 1) / L) < kMinBlocks) L >>= 1;
+  if (lines_cap > 0) {
+    // tuned lines per tile (power of two)
+    while (L > lines_cap && L > 1) L >>= 1;
+  } else {
+    // enough workgroups to fill the chip when the problem is small
+    while (L > 1 && (long long)nfb * nouter * ((nlines + L - 1) / L) < kMinBlocks) L >>= 1;
+  }
   op.d.N = N;
   op.d.logN = ilog2(N);
   op.d.L = L;
@@ -867,6 +878,68 @@ int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
   return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s, true);
 }
 
+// Batched ns2d Runge-Kutta stage over the nb runs of a batch plan: the
+// same three passes as do_ns2d_rk with the run as the outer index of the
+// y-passes (nouter = nb) and nb * ny rows in the row pass; each run's dt
+// and viscosities come from d_bpar.  Fixed dt (no CFL), no forcing.
+template <typename T>
+int do_ns2d_batch_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp, void* t1,
+                     double beta, hipStream_t s) {
+  const Geo g = geo<T>(p, true);
+  const int nb = p->nb;
+  const long long ke = (long long)kelems(p);      // modes of one run
+  const long long fs2 = (long long)nb * p->ny * g.ld2;  // W2 field stride
+  OpYInv2NS<T> op;
+  op.in = reinterpret_cast<const cpx<T>*>(stage == 0 ? S : t1);
+  op.out = W2<T>(p);
+  op.ofs = fs2;
+  op.ips = ke;
+  op.ops = (long long)p->ny * g.ld2;
+  op.ny = p->ny;
+  op.nkx = p->nkx;
+  op.ld2 = g.ld2;
+  op.ym = g.ym;
+  op.dkx = (T)p->dkx;
+  op.dky = (T)p->dky;
+  note(K_YINV, (double)nb * g.mx * (g.ym.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
+  int rc = launch_strided<T, +1>(op, p->ny, nb, g.mx, 4, p->tw[1], s, true, p->blines);
+  if (rc) return rc;
+  const int rows = nb * p->ny;
+  const double npts = (double)p->nx * p->ny;
+  note(K_XPASS, 5.0 * rows * g.mx * sizeof(cpx<T>));
+  XRNS2D<T> xr;
+  xr.beta = (T)beta;
+  rc = launch_xrow<T>(p->nx, rows, g.mx, g.ld2, g.mx, g.ld2, fs2, fs2, T(1.0 / npts),
+                      W2<T>(p), W2<T>(p), p->tw[0], nullptr, xr, s);
+  if (rc == 1) {
+    OpXNS2D<T, kXThr> ox;
+    ox.in = W2<T>(p);
+    ox.out = W2<T>(p);
+    ox.maxima = nullptr;
+    ox.beta = (T)beta;
+    rc = launch_xpass<T>(ox, p->nx, rows, 4, 1, g.mx, g.ld2, g.mx, g.ld2, fs2, fs2,
+                         T(1.0 / npts), p->tw[0], s);
+  }
+  if (rc) return rc;
+  OpYFwd2RK<T> ob;
+  ob.in = W2<T>(p);
+  ob.ny = p->ny;
+  ob.nkx = p->nkx;
+  ob.ld2 = g.ld2;
+  ob.ym = g.ym;
+  ob.mask = p->d_mask;
+  ob.ips = (long long)p->ny * g.ld2;
+  ob.ops = ke;
+  ob.bpar = p->d_bpar;
+  const double visc0[4] = {0, 0, 0, 0};
+  // state arrays hold the nb runs one after the other (one variable)
+  ob.rk = make_rk<T>(p, scheme, stage, 1, S, tmp, t1, nullptr, 0, 0, visc0);
+  ob.rk.fc = nullptr;
+  note(K_ZFWD, (double)nb * (g.mx * p->ny * sizeof(cpx<T>) +
+                             rk_bytes(p, scheme, stage, 1, sizeof(cpx<T>))));
+  return launch_strided<T, -1>(ob, p->ny, nb, g.mx, 1, p->tw[1], s, true, p->blines);
+}
+
 template <typename T>
 int do_ns3d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
                void* t1, int has_f, double fcor, int project, int no_vz_kz0,
@@ -1133,6 +1206,7 @@ int fsx_plan_destroy(fsx_plan p) {
   if (p->d_ff1) (void)hipFree(p->d_ff1);
   if (p->d_dpart) (void)hipFree(p->d_dpart);
   if (p->d_dout) (void)hipFree(p->d_dout);
+  if (p->d_bpar) (void)hipFree(p->d_bpar);
   delete p;
   return 0;
 }
@@ -1952,6 +2026,103 @@ int fsx_ns2d_budget_fields(fsx_plan p, const void* rot_fft, double beta, void* w
                         : ns2d_budget<double>(p, rot_fft, beta, work_k, work_x, out_fft, s);
   if (rc) return rc;
   return fsx_dealias(p, out_fft, 3, 0, stream);
+}
+
+
+int fsx_plan_create_batch(fsx_plan* out, int nx, int ny, double Lx, double Ly,
+                          int precision, const unsigned char* where_dealiased,
+                          int nbatch) {
+  if (!out) return fail(FSX_ERR_ARG, "null plan pointer");
+  *out = nullptr;
+  if (nbatch < 1) return fail(FSX_ERR_ARG, "nbatch must be >= 1");
+  fsx_plan p = nullptr;
+  int rc = fsx_plan_create(&p, 2, nx, ny, 1, Lx, Ly, 1.0, precision, where_dealiased, 4);
+  if (rc) return rc;
+  // the batched passes use W2 for 4 fields of every run
+  const size_t need = p->w2_elems * (size_t)nbatch;
+  (void)hipFree(p->w2);
+  p->w2 = nullptr;
+  hipError_t e = hipMalloc(&p->w2, need * 2 * (size_t)precision);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->d_bpar, 5 * sizeof(double) * nbatch);
+  if (e == hipSuccess) e = hipMemset(p->d_bpar, 0, 5 * sizeof(double) * nbatch);
+  if (e != hipSuccess) {
+    fsx_plan_destroy(p);
+    return fail(FSX_ERR_HIP, std::string("batch workspace: ") + hipGetErrorString(e));
+  }
+  p->w2_elems = need;
+  p->nb = nbatch;
+  *out = p;
+  return 0;
+}
+
+int fsx_batch_set_params(fsx_plan p, const double* dt, const double* visc4) {
+  if (!p || !dt || !visc4) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->nb < 1 || !p->d_bpar) return fail(FSX_ERR_ARG, "not a batch plan");
+  std::vector<double> h(5 * (size_t)p->nb);
+  for (int r = 0; r < p->nb; ++r) {
+    if (!(dt[r] > 0)) return fail(FSX_ERR_ARG, "every run needs dt > 0");
+    h[5 * r] = dt[r];
+    for (int j = 0; j < 4; ++j) h[5 * r + 1 + j] = visc4[4 * r + j];
+  }
+  FSX_HIP(hipMemcpy(p->d_bpar, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fsx_batch_set_lines(fsx_plan p, int lines) {
+  if (!p || p->nb < 1) return fail(FSX_ERR_ARG, "not a batch plan");
+  if (lines < 0 || (lines & (lines - 1))) return fail(FSX_ERR_ARG, "lines must be 0 or a power of two");
+  p->blines = lines;
+  return 0;
+}
+
+int fsx_ns2d_batch_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp, void* t1,
+                            double beta, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !S || !t1 || (scheme == 4 && !tmp)) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->nb < 1 || !p->d_bpar) return fail(FSX_ERR_ARG, "not a batch plan");
+  if (!((scheme == 4 && stage >= 0 && stage < 4) || (scheme == 2 && stage >= 0 && stage < 2)))
+    return fail(FSX_ERR_ARG, "scheme must be 4 (stages 0-3) or 2 (stages 0-1)");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_ns2d_batch_rk<float>(p, scheme, stage, S, tmp, t1, beta, s)
+                      : do_ns2d_batch_rk<double>(p, scheme, stage, S, tmp, t1, beta, s);
+}
+
+struct fsx_graph_s {
+  hipGraphExec_t exec;
+};
+
+int fsx_graph_begin(void* stream) {
+  if (!stream) return fail(FSX_ERR_ARG, "capture needs a created stream (not the null stream)");
+  FSX_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return 0;
+}
+
+int fsx_graph_end(void* stream, fsx_graph* out) {
+  if (!stream || !out) return fail(FSX_ERR_ARG, "bad argument");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  FSX_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(FSX_ERR_HIP, std::string("graph: ") + hipGetErrorString(e));
+  fsx_graph_s* h = new fsx_graph_s();
+  h->exec = exec;
+  *out = h;
+  return 0;
+}
+
+int fsx_graph_launch(fsx_graph g, void* stream) {
+  if (!g) return fail(FSX_ERR_ARG, "null graph");
+  FSX_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return 0;
+}
+
+int fsx_graph_destroy(fsx_graph g) {
+  if (!g) return 0;
+  (void)hipGraphExecDestroy(g->exec);
+  delete g;
+  return 0;
 }
 
 }  // extern "C"

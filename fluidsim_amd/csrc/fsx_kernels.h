@@ -451,13 +451,17 @@ struct OpYInv2NS {
   const cpx<T>* in;
   cpx<T>* out;
   long long ofs;
+  // plane o of a batch of independent runs: in + o * ips, out + o * ops
+  long long ips = 0, ops = 0;
   int ny, nkx, ld2;
   AxisMap ym;
   T dkx, dky;
 
   template <int DIR, int EPT, int kT>
-  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int, int line0) const {
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
+    const cpx<T>* in = this->in + o * ips;
+    cpx<T>* out = this->out + o * ops;
     cpx<T> sr[EPT];
     auto st = [&](int c, int l, int p, cpx<T> v) {
       const int line = line0 + l;
@@ -545,7 +549,7 @@ struct OpZFwdRK3 {
       const int idx = (p * ny + iy) * nkx + line;
       cpx<T> N[3] = {s0[e], s1[e], s2[e]};
       if (mask && mask[idx]) N[0] = N[1] = N[2] = zero<T>();
-      rk_point<T, 3>(rk, dt, idx, line, iky, ksigned(p, d.N), N);
+      rk_point<T, 3>(rk, rk.visc, dt, idx, line, iky, ksigned(p, d.N), N);
     }
   }
 };
@@ -616,7 +620,7 @@ struct OpZFwdRKStrat {
       cpx<T> N[4] = {s0[e], s1[e], s2[e] + bs,
                      scale(mul_i(sb[e]), T(-1)) - scale(vzs, N2)};
       if (mask && mask[idx]) N[0] = N[1] = N[2] = N[3] = zero<T>();
-      rk_point<T, 4>(rk, dt, idx, line, iky, ksigned(p, d.N), N);
+      rk_point<T, 4>(rk, rk.visc, dt, idx, line, iky, ksigned(p, d.N), N);
     }
   }
 };
@@ -632,11 +636,27 @@ struct OpYFwd2RK {
   AxisMap ym;
   const unsigned char* mask;
   RKArgs<T> rk;
+  // plane o of a batch of independent runs: W2 rows at o * ips, state modes
+  // at o * ops, and the run's dt and viscosities at bpar[5 o .. 5 o + 4]
+  // (dt, nu_2, nu_4, nu_8, nu_m4); bpar null: rk.dt and rk.visc
+  long long ips = 0, ops = 0;
+  const double* bpar = nullptr;
 
   template <int DIR, int EPT, int kT>
-  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int, int line0) const {
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
-    const double dt = *rk.dt;
+    const cpx<T>* in = this->in + o * ips;
+    double dt;
+    Visc visc;
+    if (bpar) {
+      const double* q = bpar + 5 * o;
+      dt = q[0];
+      visc = Visc{q[1], q[2], q[3], q[4]};
+    } else {
+      dt = *rk.dt;
+      visc = rk.visc;
+    }
+    const size_t base = (size_t)o * ops;
     tile_fft<T, DIR, EPT, kT>(
         buf, tw, d.logN, d.logL,
         [&](int, int l, int p) {
@@ -649,7 +669,7 @@ struct OpYFwd2RK {
           const int idx = p * nkx + line;
           cpx<T> N[1] = {v};
           if (mask && mask[idx]) N[0] = zero<T>();
-          rk_point<T, 1>(rk, dt, idx, line, ksigned(p, ny), 0, N);
+          rk_point<T, 1>(rk, visc, dt, base + idx, line, ksigned(p, ny), 0, N);
         });
   }
 };

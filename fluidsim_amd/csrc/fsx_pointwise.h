@@ -133,12 +133,13 @@ __device__ __forceinline__ int coarse_index(int ikx, int iky, int ikz, int nkxc,
 // (fluidsim/base/time_stepping/pseudo_spect.py:51-68, 466-517, 898-984):
 // N holds the NV tendencies of the mode (projected here when a.project),
 // E = exp(-dt sigma), E2 = exp(-dt/2 sigma) in FP64.
+// (visc: the viscosities of this mode's run; a.visc except in batches)
 template <typename T, int NV>
-__device__ __forceinline__ void rk_point(const RKArgs<T>& a, double dt,
-                                         int idx, int ikx, int iky, int ikz,
+__device__ __forceinline__ void rk_point(const RKArgs<T>& a, const Visc& visc, double dt,
+                                         size_t idx, int ikx, int iky, int ikz,
                                          cpx<T>* N) {
   const Box& bx = a.box;
-  const double sig = freq_diss(bx, a.visc, ikx, iky, ikz);
+  const double sig = freq_diss(bx, visc, ikx, iky, ikz);
   const T e = T(exp(-dt * sig));
   const T e2 = T(exp(-dt / 2 * sig));
   const T c6 = T(dt / 6), c3 = T(dt / 3), c2 = T(dt / 2), c1 = T(dt);
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
       cpx<T> N[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) N[v] = a.Nt[v][idx];
-      rk_point<T, NV>(a, dt, idx, ikx, iky, ikz, N);
+      rk_point<T, NV>(a, a.visc, dt, idx, ikx, iky, ikz, N);
     }
   }
 }

@@ -31,6 +31,7 @@ extern "C" {
 #define FSX_ERR_HIP (-2)
 
 typedef struct fsx_plan_s* fsx_plan;
+typedef struct fsx_graph_s* fsx_graph;
 
 /* ABI version (FSX_ABI_VERSION). */
 int fsx_version(void);
@@ -322,6 +323,35 @@ int fsx_diag(fsx_plan p, int kind, int flags, const void* const* fields, int nfi
  * (fluidsim/solvers/ns2d/output/spect_energy_budget.py:27-57). */
 int fsx_ns2d_budget_fields(fsx_plan p, const void* rot_fft, double beta, void* work_k,
                            void* work_x, void* out_fft, void* stream);
+
+/* ---- ensembles of independent ns2d runs ---------------------------------
+ * No counterpart in the reference (one Simul per process there); the
+ * per-run algorithm is the ns2d fused RK stage (fsx_ns2d_rk_stage,
+ * fluidsim/solvers/ns2d/solver.py:111-194,
+ * fluidsim/base/time_stepping/pseudo_spect.py:466-517, 798-984) at a fixed
+ * dt per run.
+ * fsx_plan_create_batch: a 2D plan for nbatch runs of one size; state
+ * arrays hold the runs one after the other, (nbatch, ny, nx/2+1).
+ * fsx_batch_set_params (sync): per-run dt[nbatch] and
+ * visc4[4 * nbatch] = {nu_2, nu_4, nu_8, nu_m4} of each run.
+ * fsx_batch_set_lines: lines per tile of the batched y-passes (power of two;
+ * 0 = the launcher's choice), the knob the host autotuner sets.
+ * fsx_ns2d_batch_rk_stage: one RK stage (scheme 4 or 2) of every run; no
+ * CFL, no forcing, no allocation, capturable.
+ * fsx_graph_begin / fsx_graph_end: capture the calls issued on `stream`
+ * (a created stream, not the null stream) into an executable graph;
+ * fsx_graph_launch replays it on a stream; fsx_graph_destroy frees it. */
+int fsx_plan_create_batch(fsx_plan* out, int nx, int ny, double Lx, double Ly,
+                          int precision, const unsigned char* where_dealiased,
+                          int nbatch);
+int fsx_batch_set_params(fsx_plan p, const double* dt, const double* visc4);
+int fsx_batch_set_lines(fsx_plan p, int lines);
+int fsx_ns2d_batch_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp, void* t1,
+                            double beta, void* stream);
+int fsx_graph_begin(void* stream);
+int fsx_graph_end(void* stream, fsx_graph* out);
+int fsx_graph_launch(fsx_graph g, void* stream);
+int fsx_graph_destroy(fsx_graph g);
 
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its
