@@ -8,6 +8,11 @@ taken from the initial CFL condition at 0.5.  A step is one full RK4 step
 (4 fused tendency evaluations + 4 Runge-Kutta stages) with the state
 resident in HBM.
 
+``--ensemble``: the batched ns2d ensemble instead (512 independent FP32
+runs: 256 at 128^2, 192 at 256^2, 64 at 512^2, seed = run index, nu_2
+log-uniform in [1e-4, 1e-2], RK4 at each run's fixed initial-CFL dt, one
+HIP graph launch per ensemble step); the value is ensemble steps/s.
+
 Prints ONE JSON line with the metric, a ``roofline`` object for the
 dominant kernel (algorithmic bytes per launch / average launch duration,
 measured with HIP events over the timed region), the step-level roofline of
@@ -158,6 +163,151 @@ def time_diagnostics(sim, plan, torch, reps=3):
     return res
 
 
+ENSEMBLE_SPEC = ((128, 256), (256, 192), (512, 64))
+
+
+def b_alg_ns2d(n, prec):
+    """T (b_X + b_K) + R b_K per RK4 step of one ns2d run, T = 21, R = 17."""
+    bX = prec * n * n
+    bK = 2 * prec * n * (n // 2 + 1)
+    return 21 * (bX + bK) + 17 * bK
+
+
+def ensemble_runs(spec=ENSEMBLE_SPEC, seed0=0):
+    from fluidsim_amd.synthetic import ns2d_noise_rot_fft
+
+    rng = np.random.default_rng(seed0)
+    runs, k = [], 0
+    for n, count in spec:
+        for _ in range(count):
+            runs.append({"n": n, "nu_2": float(10 ** rng.uniform(-4, -2)),
+                         "rot_fft": ns2d_noise_rot_fft(n, n, seed=seed0 + k, kmax=n // 8)})
+            k += 1
+    return runs
+
+
+def cpu_baseline_ensemble(runs, steps=2):
+    """FP64 oracle: a few RK4 steps of the first run of each size, scaled
+    by the number of runs of that size."""
+    from oracle import operators as O
+    from oracle.solvers import NS2D
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    O.set_workers(cores)
+    total, parts = 0.0, []
+    for n, count in ENSEMBLE_SPEC:
+        r = next(r for r in runs if r["n"] == n)
+        sim = NS2D(O.Oper2D(n, n), nu_2=r["nu_2"], deltat0=r["deltat"])
+        sim.init_from_rotfft(r["rot_fft"])
+        sim.run(1)
+        t0 = time.perf_counter()
+        sim.run(steps)
+        sec = (time.perf_counter() - t0) / steps
+        total += sec * count
+        parts.append(f"{n}^2: {sec * 1e3:.2f} ms/step x {count}")
+    return {
+        "value": 1.0 / total, "unit": "ensemble steps/s", "cores": cores, "kind": "port",
+        "sample": f"{steps} RK4 steps of one run per size with the FP64 oracle "
+                  f"(scipy.fft workers={cores}), scaled by the run counts: " + ", ".join(parts),
+    }
+
+
+def bench_ensemble(args, torch, _lib, copy_gbps):
+    from fluidsim_amd.ensemble import EnsembleNS2D
+
+    print("[bench] building the ensemble (512 runs)", file=sys.stderr, flush=True)
+    runs = ensemble_runs()
+    with contextlib.redirect_stdout(sys.stderr):
+        ens = EnsembleNS2D(runs, precision="float32")
+    ens.step(max(args.warmup, 1))
+    ens.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens.synchronize()
+    elapsed = time.perf_counter() - t0
+    ens.check_nan()
+    steps_per_s = args.steps / elapsed
+    balg = sum(b_alg_ns2d(r["n"], 4) for r in runs)
+    # per-pass table from eager steps with live HIP events on every group
+    ms8 = [0.0] * 8
+    by8 = [0.0] * 8
+    cn8 = [0] * 8
+    m = (_lib.ctypes.c_double * 8)()
+    b = (_lib.ctypes.c_double * 8)()
+    c = (_lib.ctypes.c_longlong * 8)()
+    for g in ens.groups:
+        _lib.call("fsx_timing_enable", g.handle, 1)
+        _lib.call("fsx_timing_read", g.handle, m, b, c, 1)
+    ens.step(3, graph=False)
+    ens.synchronize()
+    for g in ens.groups:
+        _lib.call("fsx_timing_read", g.handle, m, b, c, 1)
+        _lib.call("fsx_timing_enable", g.handle, 0)
+        for k in range(8):
+            ms8[k] += m[k]
+            by8[k] += b[k]
+            cn8[k] += c[k]
+    # the 2D passes: y inverse, row pass, y forward with the RK stage
+    names = dict(enumerate(KINDS))
+    names[4] = "y-pass forward + RK stage"
+    kernels = {}
+    for k in range(7):
+        if cn8[k]:
+            kernels[names[k]] = {"ms_per_step": ms8[k] / 3,
+                                 "alg_GBps": by8[k] / (ms8[k] * 1e-3) / 1e9,
+                                 "launches_per_step": cn8[k] / 3}
+    dom = max(range(7), key=lambda k: ms8[k])
+    achieved = by8[dom] / (ms8[dom] * 1e-3) / 1e9
+    result = {
+        "metric": "ensemble steps/s (512 ns2d runs) + %HBM roofline",
+        "value": steps_per_s,
+        "unit": "ensemble RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "ns2d ensemble: 512 independent FP32 runs (256 at 128^2, 192 at "
+                        "256^2, 64 at 512^2), seed = run index, nu_2 log-uniform in "
+                        "[1e-4, 1e-2], RK4 at each run's initial-CFL(0.5) dt, 2/3 "
+                        "dealiasing, one HIP graph launch per ensemble step",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": names[dom] + " (batched)",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": None,
+            "copy_peak_measured": copy_gbps,
+            "frac_of_copy": achieved / copy_gbps,
+        },
+        "step_roofline": {
+            "B_alg_GB_per_step": balg / 1e9,
+            "achieved_GBps": balg * steps_per_s / 1e9,
+            "frac": balg * steps_per_s / 1e9 / PEAK_HBM_GBPS,
+            "frac_of_copy": balg * steps_per_s / 1e9 / copy_gbps,
+        },
+        "kernels": kernels,
+        "tuning": {str(k): v["lines"] for k, v in ens.tuning.items()},
+    }
+    runs = ens.runs  # with each run's dt
+    ens.close()
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        with contextlib.redirect_stdout(sys.stderr):
+            result["cpu_baseline"] = cpu_baseline_ensemble(runs)
+    print(json.dumps(result), flush=True)
+
+
 def read_traffic(kernel_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -176,6 +326,8 @@ def main():
     ap.add_argument("--precision", default="float32", choices=["float32", "float64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="time the batched ns2d ensemble (512 runs) instead")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first "
                          "(calibration kernel for the HBM byte counters)")
@@ -196,6 +348,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n, prec = args.n, (4 if args.precision == "float32" else 8)
     copy_gbps = measured_copy_bandwidth(torch)
+    if args.ensemble:
+        return bench_ensemble(args, torch, _lib, copy_gbps)
     if args.copy_mb:
         src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
