@@ -100,6 +100,24 @@ SIGNATURES = {
     "fsx_graph_end": [P, P],
     "fsx_graph_launch": [P, P],
     "fsx_graph_destroy": [P],
+    "fsx_cnn_probe": [P, P, P, P],
+    "fsx_cnn_conv_fwd": [P, c_int, c_int, c_int, c_int, P, P, c_int, P, P],
+    "fsx_cnn_conv_flip": [P, c_int, c_int, c_int, P, P],
+    "fsx_cnn_wgrad_workspace": [c_int, c_int, c_int, ctypes.POINTER(ctypes.c_longlong)],
+    "fsx_cnn_conv_wgrad": [
+        P, c_int, P, c_int, c_int, c_int, c_int, P, P, P, ctypes.c_longlong, P,
+    ],
+    "fsx_cnn_act": [P, ctypes.c_longlong, c_int, c_int, c_int, P, P],
+    "fsx_cnn_act_bwd": [P, c_int, P, ctypes.c_longlong, c_int, P, P],
+    "fsx_cnn_mse": [P, P, ctypes.c_longlong, c_int, P, P, P, P],
+    "fsx_cnn_adam": [
+        P, P, P, P, c_int, c_double, c_double, c_double, c_double, c_int, P, P,
+    ],
+    "fsx_cnn_to_bf16": [P, c_int, P, P],
+    "fsx_cnn_forward": [
+        c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), P, P, P, c_int, c_int, c_int,
+        P, P, ctypes.c_longlong, P,
+    ],
     "fsx_timing_enable": [P, c_int],
     "fsx_timing_read": [
         P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),

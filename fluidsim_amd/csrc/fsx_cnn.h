@@ -1,0 +1,339 @@
+// Convolutional subgrid closure for coarse 2D LES on the matrix cores.
+//
+// Network: 5x5 periodic convolutions (the domain is periodic), 32 channels,
+// GELU between layers; BF16 weights and activations, FP32 accumulation on
+// MFMA (v_mfma_f32_32x32x8_bf16), FP32 pre-activations, FP32 master
+// weights with Adam.
+//
+// Layouts (channels last):
+//   activations  x[B][H][W][C]  BF16, C = 8 or 32 (channel-padded with 0)
+//   pre-act.     y[B][H][W][co] FP32
+//   weights      w[32][25][C]   BF16 (output channel, tap dy*5+dx, input
+//                channel), rows beyond the layer's outputs are zero
+//   bias         b[32]          FP32
+// Forward:  y[p][o] = b[o] + sum_{t, c} w[o][t][c] x[p + d(t)][c],
+//           d(t) = (dy - 2, dx - 2) with periodic wrap.
+// Input gradient: the same kernel on dy with the flipped, transposed
+// weights wt[c][24 - t][o] = w[o][t][c].
+// Weight gradient: dw[o][t][c] = sum_p dy[p][o] x[p + d(t)][c].
+//
+// MFMA 32x32x8 BF16 operand layout (64 lanes, m = lane % 32,
+// g = lane / 32): lane supplies A[m][4g .. 4g+3] and B[4g .. 4g+3][m];
+// it receives D[8 (i / 4) + 4 g + i % 4][m], i = 0..15.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsx {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kCnnTaps = 25;
+constexpr int kCnnRows = 4;  // output rows per workgroup of the convolutions
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  // round to nearest even (finite inputs)
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+
+__device__ __forceinline__ f32x16 mfma_bf16(bf16x4 a, bf16x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// D (32 x 32, row-major FP32) = A (32 x 8 BF16, row-major) x B (8 x 32 BF16,
+// row-major): one wave; checks the operand layout above.
+__global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint16_t* B,
+                                                   float* D) {
+  const int lane = threadIdx.x, m = lane & 31, g = lane >> 5;
+  bf16x4 a, b;
+  for (int j = 0; j < 4; ++j) {
+    a[j] = (short)A[m * 8 + 4 * g + j];
+    b[j] = (short)B[(4 * g + j) * 32 + m];
+  }
+  f32x16 acc = {};
+  acc = mfma_bf16(a, b, acc);
+  for (int i = 0; i < 16; ++i) D[(8 * (i >> 2) + 4 * g + (i & 3)) * 32 + m] = acc[i];
+}
+
+__host__ __device__ constexpr int cnn_kp(int ci) { return kCnnTaps * ci + 4; }
+__host__ __device__ constexpr int cnn_ps(int ci) { return ci + 4; }
+
+// LDS bytes of k_conv_fwd: weights [32][KP] + patch [rows + 4][W + 4][PS]
+__host__ __device__ inline size_t conv_fwd_lds(int ci, int W) {
+  return (size_t)2 * (32 * cnn_kp(ci) + (size_t)(kCnnRows + 4) * (W + 4) * cnn_ps(ci));
+}
+
+// One workgroup (4 waves) per (image, kCnnRows output rows); each wave
+// computes 32 output channels x 32 pixels tiles.
+template <int CI>
+__global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
+                                                  const uint16_t* __restrict__ w,
+                                                  const float* __restrict__ bias,
+                                                  float* __restrict__ y, int H, int W,
+                                                  int co_out) {
+  constexpr int K = kCnnTaps * CI;
+  constexpr int KP = cnn_kp(CI);
+  constexpr int PS = cnn_ps(CI);
+  extern __shared__ __align__(16) uint16_t cnn_sm[];
+  uint16_t* sw = cnn_sm;
+  uint16_t* sp = cnn_sm + 32 * KP;
+  const int WP = W + 4;
+  const int nyb = H / kCnnRows;
+  const int b = blockIdx.x / nyb, y0 = (blockIdx.x % nyb) * kCnnRows;
+  for (int t = threadIdx.x; t < 32 * K / 4; t += blockDim.x) {
+    const int co = (4 * t) / K, k = (4 * t) % K;
+    *reinterpret_cast<uint2*>(&sw[co * KP + k]) =
+        *reinterpret_cast<const uint2*>(&w[co * K + k]);
+  }
+  constexpr int NV = CI / 4;  // 8-byte vectors per pixel
+  const int nrow = kCnnRows + 4;
+  for (int t = threadIdx.x; t < nrow * WP * NV; t += blockDim.x) {
+    const int v = t % NV, rest = t / NV;
+    const int c = rest % WP, r = rest / WP;
+    const int gy = (y0 - 2 + r + H) % H;
+    const int gx = (c - 2 + W) % W;
+    *reinterpret_cast<uint2*>(&sp[(r * WP + c) * PS + 4 * v]) =
+        *reinterpret_cast<const uint2*>(&x[(((size_t)b * H + gy) * W + gx) * CI + 4 * v]);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int m = lane & 31, g = lane >> 5;
+  const int nchunk = W / 32;
+  for (int tile = wid; tile < kCnnRows * nchunk; tile += nw) {
+    const int r = tile / nchunk, n0 = (tile % nchunk) * 32;
+    f32x16 acc = {};
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      const int k = k0 + 4 * g;
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&sw[m * KP + k]);
+      const int tap = k / CI, ci = k % CI;
+      const int dy = tap / 5, dx = tap % 5;
+      const bf16x4 bb =
+          *reinterpret_cast<const bf16x4*>(&sp[((r + dy) * WP + n0 + m + dx) * PS + ci]);
+      acc = mfma_bf16(a, bb, acc);
+    }
+    float* yo = y + (((size_t)b * H + y0 + r) * W + n0 + m) * co_out;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int co = 8 * (i >> 2) + 4 * g + (i & 3);
+      if (co < co_out) yo[co] = acc[i] + bias[co];
+    }
+  }
+}
+
+// wt[c][24 - t][o] = w[o][t][c] for o < co, c < ci; wt has 32 rows and
+// CO_PAD input channels (zero-padded).
+__global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad, uint16_t* wt) {
+  const int n = 32 * kCnnTaps * co_pad;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int o = i % co_pad, t = (i / co_pad) % kCnnTaps, c = i / (co_pad * kCnnTaps);
+    uint16_t v = 0;
+    if (c < ci && o < co) v = w[((size_t)o * kCnnTaps + (24 - t)) * ci + c];
+    wt[i] = v;
+  }
+}
+
+// LDS bytes of k_conv_wgrad: dy rows [rows][32][W + 5] + x patch
+// [rows + 4][CI][W + 9] (BF16; odd pitches spread the 16-bit reads of
+// neighbouring lanes over the banks).
+__host__ __device__ inline size_t conv_wgrad_lds(int ci, int W) {
+  return (size_t)2 * ((size_t)kCnnRows * 32 * (W + 5) + (size_t)(kCnnRows + 4) * ci * (W + 9));
+}
+
+// Weight gradient, persistent: workgroup g takes the row blocks
+// g, g + G, ... of all images, accumulates dw in MFMA registers and writes
+// its partial part[g][32][25 * CI]; k_cnn_reduce sums the partials.
+// Each wave owns the N tiles (32 columns of (tap, ci)) wid, wid + 4, ...
+template <int CI, int CO_IN>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x,
+                                                    const uint16_t* __restrict__ dy,
+                                                    float* __restrict__ part, int B, int H,
+                                                    int W) {
+  constexpr int N = kCnnTaps * CI;
+  constexpr int NT = (N + 31) / 32;
+  constexpr int TPW = (NT + 3) / 4;  // tiles per wave
+  extern __shared__ __align__(16) uint16_t cnn_sm[];
+  const int DP = W + 5, XP = W + 9;
+  uint16_t* sd = cnn_sm;                            // [rows][32][DP]
+  uint16_t* sx = cnn_sm + kCnnRows * 32 * DP;       // [rows + 4][CI][XP]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int m = lane & 31, g = lane >> 5;
+  f32x16 acc[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) acc[q] = f32x16{};
+  const int nyb = H / kCnnRows;
+  const int nblk = B * nyb;
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int b = blk / nyb, y0 = (blk % nyb) * kCnnRows;
+    __syncthreads();
+    // dy rows, transposed to [r][o][x]; channels beyond CO_IN are zero
+    for (int t = threadIdx.x; t < kCnnRows * W * 32; t += blockDim.x) {
+      const int o = t % 32, rest = t / 32, px = rest % W, r = rest / W;
+      uint16_t v = 0;
+      if (o < CO_IN) v = dy[(((size_t)b * H + y0 + r) * W + px) * CO_IN + o];
+      sd[(r * 32 + o) * DP + px] = v;
+    }
+    // x rows y0 - 2 .. y0 + rows + 1, columns -2 .. W + 1, transposed
+    for (int t = threadIdx.x; t < (kCnnRows + 4) * (W + 4) * CI; t += blockDim.x) {
+      const int c = t % CI, rest = t / CI, col = rest % (W + 4), r = rest / (W + 4);
+      const int gy = (y0 - 2 + r + H) % H, gx = (col - 2 + W) % W;
+      sx[(r * CI + c) * XP + col] = x[(((size_t)b * H + gy) * W + gx) * CI + c];
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < kCnnRows * W; k0 += 8) {
+      // 8 consecutive pixels of one row (W is a multiple of 8)
+      const int r = k0 / W, px0 = k0 % W + 4 * g;
+      bf16x4 a;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[j] = (short)sd[(r * 32 + m) * DP + px0 + j];
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) {
+        const int tile = wid + 4 * q;
+        if (tile >= NT) break;
+        const int n = tile * 32 + m;
+        bf16x4 bb = {0, 0, 0, 0};
+        if (n < N) {
+          const int tap = n / CI, c = n % CI;
+          const int tdy = tap / 5, tdx = tap % 5;
+          const uint16_t* src = &sx[((r + tdy) * CI + c) * XP + px0 + tdx];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) bb[j] = (short)src[j];
+        }
+        acc[q] = mfma_bf16(a, bb, acc[q]);
+      }
+    }
+  }
+  float* dst = part + (size_t)blockIdx.x * 32 * N;
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int tile = wid + 4 * q;
+    if (tile >= NT) break;
+    const int n = tile * 32 + m;
+    if (n >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int o = 8 * (i >> 2) + 4 * g + (i & 3);
+      dst[(size_t)o * N + n] = acc[q][i];
+    }
+  }
+}
+
+// out[i] = sum_g part[g][i] (fixed order), i < n
+__global__ void k_cnn_reduce(const float* part, int ngroups, int n, float* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int gq = 0; gq < ngroups; ++gq) s += part[(size_t)gq * n + i];
+    out[i] = s;
+  }
+}
+
+// bias gradient partials: part[blk][o] = sum over the block's pixels of dy[p][o]
+__global__ __launch_bounds__(256) void k_cnn_bias_grad(const uint16_t* dy, long long npix,
+                                                       int co, float* part) {
+  __shared__ float red[256];
+  const int o = threadIdx.x % 32, lane8 = threadIdx.x / 32;
+  float s = 0.0f;
+  if (o < co)
+    for (long long p = blockIdx.x * 8LL + lane8; p < npix; p += gridDim.x * 8LL)
+      s += from_bf16(dy[p * co + o]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    float t = 0.0f;
+    for (int j = 0; j < 8; ++j) t += red[j * 32 + threadIdx.x];
+    part[blockIdx.x * 32 + threadIdx.x] = t;
+  }
+}
+
+__device__ __forceinline__ float gelu(float v) {
+  return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float v) {
+  const float cdf = 0.5f * (1.0f + erff(v * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * v * v);
+  return cdf + v * pdf;
+}
+
+// a[p][c] = BF16(act(y[p][c])) for c < cin, 0 up to cpad; act = GELU
+// (gelu != 0) or identity
+__global__ void k_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu,
+                          uint16_t* a) {
+  const long long n = npix * cpad;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / cpad;
+    const int c = (int)(i % cpad);
+    float v = 0.0f;
+    if (c < cin) {
+      v = y[p * cin + c];
+      if (use_gelu) v = gelu(v);
+    }
+    a[i] = to_bf16(v);
+  }
+}
+
+// d[p][c] = BF16(g[p][c] * gelu'(y[p][c])) (g: gradient w.r.t. the GELU
+// output, channel stride gs; y: pre-activation, c channels)
+__global__ void k_cnn_act_bwd(const float* g, int gs, const float* y, long long npix, int c,
+                              uint16_t* d) {
+  const long long n = npix * c;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / c;
+    const int k = (int)(i % c);
+    d[i] = to_bf16(g[p * gs + k] * gelu_grad(y[i]));
+  }
+}
+
+// loss partials and gradient of L = mean((out - target)^2):
+// d[p][c] = BF16(2 (out - target) / n) for c = 0, 0 up to cpad
+__global__ __launch_bounds__(256) void k_cnn_mse(const float* out, const float* target,
+                                                 long long npix, int cpad, uint16_t* d,
+                                                 double* part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  const double inv = 1.0 / (double)npix;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < npix;
+       p += (long long)gridDim.x * blockDim.x) {
+    const float e = out[p] - target[p];
+    s += (double)e * e;
+    d[p * cpad] = to_bf16((float)(2.0 * inv) * e);
+    for (int c = 1; c < cpad; ++c) d[p * cpad + c] = 0;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] * inv;
+}
+
+// Adam (FP32 master weights) and the BF16 copy used by the kernels
+__global__ void k_cnn_adam(float* w, const float* gr, float* m1, float* m2, int n, float lr,
+                           float b1, float b2, float eps, float c1, float c2, uint16_t* wb) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gi = gr[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    const float wi = w[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = to_bf16(wi);
+  }
+}
+
+// BF16 copy of FP32 values (weights)
+__global__ void k_cnn_to_bf16(const float* in, int n, uint16_t* out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = to_bf16(in[i]);
+}
+
+}  // namespace fsx

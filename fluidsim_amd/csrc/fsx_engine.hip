@@ -22,6 +22,7 @@
 #include "fsx_forcing.h"
 #include "fsx_turb.h"
 #include "fsx_diag.h"
+#include "fsx_cnn.h"
 
 using namespace fsx;
 
@@ -2122,6 +2123,216 @@ int fsx_graph_destroy(fsx_graph g) {
   if (!g) return 0;
   (void)hipGraphExecDestroy(g->exec);
   delete g;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+bool cnn_channels_ok(int c) { return c == 8 || c == 32; }
+
+int cnn_grid(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)std::max<long long>(1, std::min<long long>(b, 4096));
+}
+
+int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, const float* bias,
+                 int co_out, float* y, hipStream_t s) {
+  if (!x || !w || !bias || !y || B < 1) return fail(FSX_ERR_ARG, "bad argument");
+  if (!cnn_channels_ok(ci)) return fail(FSX_ERR_ARG, "input channels must be 8 or 32");
+  if (co_out < 1 || co_out > 32) return fail(FSX_ERR_ARG, "output channels must be 1..32");
+  if (H < kCnnRows || H % kCnnRows || W < 32 || W % 32)
+    return fail(FSX_ERR_ARG, "H must be a multiple of 4 and W a multiple of 32");
+  const size_t lds = conv_fwd_lds(ci, W);
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "image too wide for the convolution tile");
+  const unsigned nblk = (unsigned)(B * (H / kCnnRows));
+  const auto* xi = reinterpret_cast<const uint16_t*>(x);
+  const auto* wi = reinterpret_cast<const uint16_t*>(w);
+  int rc;
+  if (ci == 8) {
+    if ((rc = set_lds(k_conv_fwd<8>, lds))) return rc;
+    hipLaunchKernelGGL(k_conv_fwd<8>, dim3(nblk), dim3(256), lds, s, xi, wi, bias, y, H, W,
+                       co_out);
+  } else {
+    if ((rc = set_lds(k_conv_fwd<32>, lds))) return rc;
+    hipLaunchKernelGGL(k_conv_fwd<32>, dim3(nblk), dim3(256), lds, s, xi, wi, bias, y, H, W,
+                       co_out);
+  }
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,
+            hipStream_t s) {
+  if (!y || !a || npix < 1 || cin < 1 || cpad < cin) return fail(FSX_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(k_cnn_act, dim3(cnn_grid(npix * cpad)), dim3(256), 0, s, y, npix, cin,
+                     cpad, use_gelu, reinterpret_cast<uint16_t*>(a));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsx_cnn_probe(const void* A, const void* B, float* D, void* stream) {
+  if (!A || !B || !D) return fail(FSX_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                     reinterpret_cast<const uint16_t*>(A), reinterpret_cast<const uint16_t*>(B),
+                     D);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w,
+                     const float* bias, int co_out, float* y, void* stream) {
+  return cnn_conv_fwd(x, B, H, W, ci, w, bias, co_out, y, (hipStream_t)stream);
+}
+
+int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void* stream) {
+  if (!w || !wt || ci < 1 || ci > 32 || co < 1 || co > co_pad || !cnn_channels_ok(co_pad))
+    return fail(FSX_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(k_conv_flip, dim3(cnn_grid(32LL * kCnnTaps * co_pad)), dim3(256), 0,
+                     (hipStream_t)stream, reinterpret_cast<const uint16_t*>(w), ci, co, co_pad,
+                     reinterpret_cast<uint16_t*>(wt));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_wgrad_workspace(int B, int H, int ci, long long* bytes) {
+  if (!bytes || B < 1 || H < kCnnRows || !cnn_channels_ok(ci)) return fail(FSX_ERR_ARG, "bad argument");
+  const long long nblk = (long long)B * (H / kCnnRows);
+  const long long groups = std::min<long long>(nblk, 256);
+  // weight partials, then bias partials
+  *bytes = groups * 32LL * kCnnTaps * ci * 4 + 1024LL * 32 * 4;
+  return 0;
+}
+
+int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W,
+                       float* dw, float* dbias, void* work, long long work_bytes,
+                       void* stream) {
+  if (!x || !dy || !dw || !dbias || !work || B < 1) return fail(FSX_ERR_ARG, "bad argument");
+  if (!cnn_channels_ok(ci) || !cnn_channels_ok(co_in))
+    return fail(FSX_ERR_ARG, "channels must be 8 or 32");
+  if (H < kCnnRows || H % kCnnRows || W < 8 || W % 8)
+    return fail(FSX_ERR_ARG, "H must be a multiple of 4 and W of 8");
+  long long need = 0;
+  fsx_cnn_wgrad_workspace(B, H, ci, &need);
+  if (work_bytes < need) return fail(FSX_ERR_ARG, "workspace too small");
+  const size_t lds = conv_wgrad_lds(ci, W);
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "image too wide for the weight-gradient tile");
+  hipStream_t s = (hipStream_t)stream;
+  const long long nblk = (long long)B * (H / kCnnRows);
+  const int groups = (int)std::min<long long>(nblk, 256);
+  float* part = reinterpret_cast<float*>(work);
+  const auto* xi = reinterpret_cast<const uint16_t*>(x);
+  const auto* di = reinterpret_cast<const uint16_t*>(dy);
+  int rc;
+#define FSX_WG(CI_, CO_)                                                              \
+  if ((rc = set_lds(k_conv_wgrad<CI_, CO_>, lds))) return rc;                          \
+  hipLaunchKernelGGL((k_conv_wgrad<CI_, CO_>), dim3(groups), dim3(256), lds, s, xi, di, \
+                     part, B, H, W);
+  if (ci == 8 && co_in == 8) {
+    FSX_WG(8, 8)
+  } else if (ci == 8) {
+    FSX_WG(8, 32)
+  } else if (co_in == 8) {
+    FSX_WG(32, 8)
+  } else {
+    FSX_WG(32, 32)
+  }
+#undef FSX_WG
+  FSX_HIP(hipGetLastError());
+  const int n = 32 * kCnnTaps * ci;
+  hipLaunchKernelGGL(k_cnn_reduce, dim3(cnn_grid(n)), dim3(256), 0, s, part, groups, n, dw);
+  FSX_HIP(hipGetLastError());
+  float* bpart = part + (size_t)groups * n;
+  const long long npix = (long long)B * H * W;
+  hipLaunchKernelGGL(k_cnn_bias_grad, dim3(1024), dim3(256), 0, s, di, npix, co_in, bpart);
+  FSX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cnn_reduce, dim3(1), dim3(256), 0, s, bpart, 1024, 32, dbias);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,
+                void* stream) {
+  return cnn_act(y, npix, cin, cpad, use_gelu, a, (hipStream_t)stream);
+}
+
+int fsx_cnn_act_bwd(const float* g, int gs, const float* y, long long npix, int c, void* d,
+                    void* stream) {
+  if (!g || !y || !d || npix < 1 || c < 1 || gs < c) return fail(FSX_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(k_cnn_act_bwd, dim3(cnn_grid(npix * c)), dim3(256), 0,
+                     (hipStream_t)stream, g, gs, y, npix, c, reinterpret_cast<uint16_t*>(d));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_mse(const float* out, const float* target, long long npix, int cpad, void* d,
+                double* work, double* host_loss, void* stream) {
+  if (!out || !target || !d || !work || npix < 1 || cpad < 1) return fail(FSX_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cnn_mse, dim3(256), dim3(256), 0, s, out, target, npix, cpad,
+                     reinterpret_cast<uint16_t*>(d), work);
+  FSX_HIP(hipGetLastError());
+  if (host_loss) {
+    double h[256];
+    FSX_HIP(hipMemcpyAsync(h, work, sizeof(h), hipMemcpyDeviceToHost, s));
+    FSX_HIP(hipStreamSynchronize(s));
+    double t = 0.0;
+    for (int i = 0; i < 256; ++i) t += h[i];
+    *host_loss = t;
+  }
+  return 0;
+}
+
+int fsx_cnn_adam(float* w, const float* g, float* m1, float* m2, int n, double lr, double b1,
+                 double b2, double eps, int step, void* wb, void* stream) {
+  if (!w || !g || !m1 || !m2 || n < 1 || step < 1) return fail(FSX_ERR_ARG, "bad argument");
+  const float c1 = (float)(1.0 - std::pow(b1, step)), c2 = (float)(1.0 - std::pow(b2, step));
+  hipLaunchKernelGGL(k_cnn_adam, dim3(cnn_grid(n)), dim3(256), 0, (hipStream_t)stream, w, g,
+                     m1, m2, n, (float)lr, (float)b1, (float)b2, (float)eps, c1, c2,
+                     reinterpret_cast<uint16_t*>(wb));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_to_bf16(const float* in, int n, void* out, void* stream) {
+  if (!in || !out || n < 1) return fail(FSX_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(k_cnn_to_bf16, dim3(cnn_grid(n)), dim3(256), 0, (hipStream_t)stream, in,
+                     n, reinterpret_cast<uint16_t*>(out));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
+                    const float* const* b, const void* x, int B, int H, int W, float* y,
+                    void* work, long long work_bytes, void* stream) {
+  if (nl < 1 || nl > 16 || !cin || !cout || !w || !b || !x || !y || !work)
+    return fail(FSX_ERR_ARG, "bad argument");
+  const long long npix = (long long)B * H * W;
+  // two BF16 activation buffers of 32 channels and one FP32 pre-activation
+  const long long need = 2 * npix * 32 * 2 + npix * 32 * 4;
+  if (work_bytes < need) return fail(FSX_ERR_ARG, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  uint16_t* act[2] = {reinterpret_cast<uint16_t*>(work),
+                      reinterpret_cast<uint16_t*>(work) + npix * 32};
+  float* pre = reinterpret_cast<float*>(reinterpret_cast<char*>(work) + 2 * npix * 32 * 2);
+  const void* in = x;
+  for (int l = 0; l < nl; ++l) {
+    const bool last = l == nl - 1;
+    if (!cnn_channels_ok(cin[l])) return fail(FSX_ERR_ARG, "layer input channels must be 8 or 32");
+    if (!last && cout[l] != 32) return fail(FSX_ERR_ARG, "hidden layers have 32 channels");
+    int rc = cnn_conv_fwd(in, B, H, W, cin[l], w[l], b[l], cout[l], last ? y : pre, s);
+    if (rc) return rc;
+    if (!last) {
+      rc = cnn_act(pre, npix, 32, 32, 1, act[l & 1], s);
+      if (rc) return rc;
+      in = act[l & 1];
+    }
+  }
   return 0;
 }
 

@@ -353,6 +353,51 @@ int fsx_graph_end(void* stream, fsx_graph* out);
 int fsx_graph_launch(fsx_graph g, void* stream);
 int fsx_graph_destroy(fsx_graph g);
 
+/* ---- convolutional subgrid closure (matrix cores) ------------------------
+ * No counterpart in the reference (fluidsim's LES hook, TurbModel.get_forcing
+ * in fluidsim/base/turb_model/base.py:87-88, has Smagorinsky only).
+ * Channels-last layouts: activations BF16 [B][H][W][C] with C = 8 or 32
+ * (zero-padded), pre-activations FP32 [B][H][W][co], weights BF16
+ * [32][25][C] (tap t = 5 dy + dx, offset (dy - 2, dx - 2), periodic),
+ * bias FP32 [32]; MFMA 32x32x8 BF16 with FP32 accumulation.
+ * fsx_cnn_probe: D (32x32 FP32) = A (32x8 BF16) B (8x32 BF16), one wave.
+ * fsx_cnn_conv_fwd: y = bias + periodic 5x5 correlation of x, co_out <= 32
+ *   outputs; H multiple of 4, W multiple of 32, W <= 128 for C = 32.
+ * fsx_cnn_conv_flip: wt[c][24 - t][o] = w[o][t][c] (co_pad = 8 or 32
+ *   input channels of wt), so fsx_cnn_conv_fwd(dy, wt) is the input gradient.
+ * fsx_cnn_conv_wgrad: dw[o][t][c] = sum_p dy[p][o] x[p + d(t)][c] and
+ *   dbias[o] = sum_p dy[p][o] (FP32), workspace from fsx_cnn_wgrad_workspace;
+ *   deterministic (fixed-order reductions).
+ * fsx_cnn_act: a = BF16(GELU(y)) (gelu = 1) or BF16(y), channel-padded.
+ * fsx_cnn_act_bwd: d = BF16(g * GELU'(y)).
+ * fsx_cnn_mse: loss = mean((out - target)^2) (host_loss: sync readback,
+ *   NULL: none) and d = BF16(2 (out - target) / n) in channel 0 of cpad.
+ * fsx_cnn_adam: Adam step of n FP32 values (bias-corrected, step >= 1),
+ *   optional BF16 copy into wb.
+ * fsx_cnn_to_bf16: BF16 copy of n FP32 values.
+ * fsx_cnn_forward: nl layers (GELU between them) in one call, inference;
+ *   work holds 2 x B H W x 32 BF16 + B H W x 32 FP32. */
+int fsx_cnn_probe(const void* A, const void* B, float* D, void* stream);
+int fsx_cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w,
+                     const float* bias, int co_out, float* y, void* stream);
+int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void* stream);
+int fsx_cnn_wgrad_workspace(int B, int H, int ci, long long* bytes);
+int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W,
+                       float* dw, float* dbias, void* work, long long work_bytes,
+                       void* stream);
+int fsx_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,
+                void* stream);
+int fsx_cnn_act_bwd(const float* g, int gs, const float* y, long long npix, int c, void* d,
+                    void* stream);
+int fsx_cnn_mse(const float* out, const float* target, long long npix, int cpad, void* d,
+                double* work, double* host_loss, void* stream);
+int fsx_cnn_adam(float* w, const float* g, float* m1, float* m2, int n, double lr, double b1,
+                 double b2, double eps, int step, void* wb, void* stream);
+int fsx_cnn_to_bf16(const float* in, int n, void* out, void* stream);
+int fsx_cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
+                    const float* const* b, const void* x, int B, int H, int W, float* y,
+                    void* work, long long work_bytes, void* stream);
+
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its
  * stream.  fsx_timing_read (sync) returns, per launch kind

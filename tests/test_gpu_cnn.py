@@ -1,0 +1,174 @@
+"""GPU tests of the closure network kernels (fsx_cnn_*) against the FP64
+oracle (oracle/cnn.py) on the same BF16-rounded inputs and weights.
+
+Bars: the MFMA operand layout is exact on small integers; convolutions,
+input and weight gradients (FP32 accumulation of <= 800-term BF16
+products) within 1e-5 of the largest value; the network forward within
+5e-3 relative L2 of an oracle that rounds its activations to BF16 like the
+kernels; the gradients of one training step (which pass through BF16)
+within 3e-2 relative L2 of the FP64 gradients; training lowers the loss.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cnn as C
+from fluidsim_amd import _lib
+from fluidsim_amd.closure import CNNClosure, bf16_bits, bf16_value
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to("cuda") if dtype is None else t.to("cuda", dtype)
+
+
+def rounded(a):
+    return bf16_value(bf16_bits(a)).astype(np.float64)
+
+
+def relmax(got, ref):
+    return np.abs(got - ref).max() / np.abs(ref).max()
+
+
+def test_mfma_layout_probe():
+    rng = np.random.default_rng(0)
+    A = rng.integers(-4, 5, (32, 8)).astype(np.float32)
+    B = rng.integers(-4, 5, (8, 32)).astype(np.float32)
+    D = torch.zeros((32, 32), dtype=torch.float32, device="cuda")
+    # device buffers are held in locals: a temporary freed after dptr() would
+    # hand its memory to the next allocation before the kernel runs
+    a, b = dev(bf16_bits(A)), dev(bf16_bits(B))
+    _lib.call("fsx_cnn_probe", _lib.dptr(a), _lib.dptr(b), _lib.dptr(D), _lib.stream_ptr())
+    assert np.array_equal(D.cpu().numpy(), A @ B)
+
+
+def conv_gpu(x, w, b, co_out):
+    B_, H, W, ci = x.shape
+    y = torch.empty((B_, H, W, co_out), dtype=torch.float32, device="cuda")
+    xd, wd, bd = dev(bf16_bits(x)), dev(bf16_bits(w)), dev(b.astype(np.float32))
+    _lib.call("fsx_cnn_conv_fwd", _lib.dptr(xd), B_, H, W, ci, _lib.dptr(wd), _lib.dptr(bd),
+              co_out, _lib.dptr(y), _lib.stream_ptr())
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("ci,shape", [(8, (2, 8, 32)), (32, (2, 8, 32)), (32, (1, 16, 128)),
+                                      (8, (3, 12, 64))])
+def test_conv_forward(ci, shape):
+    rng = np.random.default_rng(1)
+    x = rounded(rng.standard_normal(shape + (ci,)))
+    w = rounded(rng.standard_normal((32, 25, ci)) * 0.2)
+    b = rng.standard_normal(32)
+    for co in (32, 1):
+        got = conv_gpu(x, w, b, co)
+        ref = C.conv(x, w, b.astype(np.float32).astype(np.float64), co)
+        print(f"conv ci={ci} {shape} co={co}: {relmax(got, ref):.2e}")
+        assert relmax(got, ref) < 1e-5
+
+
+@pytest.mark.parametrize("ci,co,co_pad", [(32, 32, 32), (32, 1, 8), (8, 32, 32)])
+def test_input_gradient_by_flip(ci, co, co_pad):
+    rng = np.random.default_rng(2)
+    B_, H, W = 2, 8, 32
+    w = np.zeros((32, 25, ci))
+    w[:co] = rounded(rng.standard_normal((co, 25, ci)) * 0.2)
+    dy = np.zeros((B_, H, W, co_pad))
+    dy[..., :co] = rounded(rng.standard_normal((B_, H, W, co)))
+    wt = torch.empty((32, 25, co_pad), dtype=torch.int16, device="cuda")
+    s = _lib.stream_ptr()
+    wd = dev(bf16_bits(w))
+    _lib.call("fsx_cnn_conv_flip", _lib.dptr(wd), ci, co, co_pad, _lib.dptr(wt), s)
+    g = torch.empty((B_, H, W, ci), dtype=torch.float32, device="cuda")
+    zb = torch.zeros(32, dtype=torch.float32, device="cuda")
+    dyd = dev(bf16_bits(dy))
+    _lib.call("fsx_cnn_conv_fwd", _lib.dptr(dyd), B_, H, W, co_pad, _lib.dptr(wt),
+              _lib.dptr(zb), ci, _lib.dptr(g), s)
+    ref = C.conv_input_grad(dy[..., :co], w, ci)
+    assert relmax(g.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("ci,co_in,shape", [(8, 32, (2, 8, 32)), (32, 32, (2, 8, 32)),
+                                            (32, 8, (3, 16, 64)), (8, 8, (1, 4, 16))])
+def test_weight_gradient(ci, co_in, shape):
+    rng = np.random.default_rng(3)
+    x = rounded(rng.standard_normal(shape + (ci,)))
+    dy = rounded(rng.standard_normal(shape + (co_in,)))
+    B_, H, W = shape
+    nb = ctypes_ll()
+    _lib.call("fsx_cnn_wgrad_workspace", B_, H, ci, nb)
+    work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    dw = torch.empty((32, 25, ci), dtype=torch.float32, device="cuda")
+    db = torch.empty(32, dtype=torch.float32, device="cuda")
+    xd, dyd = dev(bf16_bits(x)), dev(bf16_bits(dy))
+    _lib.call("fsx_cnn_conv_wgrad", _lib.dptr(xd), ci, _lib.dptr(dyd), co_in, B_, H, W,
+              _lib.dptr(dw), _lib.dptr(db), _lib.dptr(work), nb.value, _lib.stream_ptr())
+    rdw, rdb = C.conv_weight_grad(x, dy)
+    got = dw.cpu().numpy()
+    assert relmax(got[:co_in], rdw) < 1e-5
+    assert np.all(got[co_in:] == 0)
+    assert relmax(db.cpu().numpy()[:co_in], rdb) < 1e-5
+
+
+def ctypes_ll():
+    import ctypes
+
+    return ctypes.c_longlong(0)
+
+
+def oracle_net(net):
+    layers = []
+    for l in range(net.nlayers):
+        w = rounded(net.w[l].cpu().numpy())
+        co = net.cout[l]
+        layers.append((w[:co], net.b[l].cpu().numpy()[:co].astype(np.float64)))
+    return C.Net(layers)
+
+
+def forward_bf16_acts(onet, x):
+    """Oracle forward rounding the activations to BF16 like the kernels."""
+    a = rounded(x)
+    for i, (w, b) in enumerate(onet.layers):
+        y = C.conv(a, w[:, :, : a.shape[3]], b)
+        a = rounded(C.gelu(y)) if i < len(onet.layers) - 1 else y
+    return a[..., 0]
+
+
+def test_network_forward():
+    rng = np.random.default_rng(4)
+    net = CNNClosure(n_in=1, nlayers=6, seed=5)
+    x = rng.standard_normal((2, 16, 64, 1)).astype(np.float32)
+    got = net.forward(dev(x)).cpu().numpy()
+    onet = oracle_net(net)
+    ref = forward_bf16_acts(onet, np.concatenate([x, np.zeros(x.shape[:3] + (7,))], axis=3))
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"network forward rel L2 = {err:.2e}")
+    assert err < 5e-3
+
+
+def test_training_step_gradients_and_descent():
+    rng = np.random.default_rng(6)
+    net = CNNClosure(n_in=1, nlayers=4, seed=7, lr=2e-3)
+    B_, H, W = 4, 16, 32
+    x = rng.standard_normal((B_, H, W, 1)).astype(np.float32)
+    # a smooth, learnable target: a local average of the input
+    xr = x[..., 0].astype(np.float64)
+    target = sum(np.roll(xr, (dy, dx), axis=(1, 2)) for dy in (-1, 0, 1) for dx in (-1, 0, 1)) / 9
+    onet = oracle_net(net)
+    xpad = np.concatenate([rounded(x), np.zeros(x.shape[:3] + (7,))], axis=3)
+    loss_ref, grads_ref, _ = onet.gradients(xpad, target)
+    loss = net.train_step(dev(x), dev(target.astype(np.float32)))
+    assert loss == pytest.approx(loss_ref, rel=5e-3)
+    gw, gb = net.last_grads
+    for l in range(net.nlayers):
+        co = net.cout[l]
+        got = gw[l].cpu().numpy()[:co, :, : grads_ref[l][0].shape[2]]
+        err = np.linalg.norm(got - grads_ref[l][0]) / np.linalg.norm(grads_ref[l][0])
+        errb = np.linalg.norm(gb[l].cpu().numpy()[:co] - grads_ref[l][1]) / np.linalg.norm(
+            grads_ref[l][1])
+        print(f"layer {l}: dw rel L2 = {err:.2e}, db rel L2 = {errb:.2e}")
+        assert err < 3e-2 and errb < 3e-2
+    losses = [net.train_step(dev(x), dev(target.astype(np.float32))) for _ in range(200)]
+    print(f"loss {loss:.3e} -> {losses[-1]:.3e}")
+    assert losses[-1] < 0.1 * loss
