@@ -50,6 +50,10 @@ SIGNATURES = {
         P, c_int, c_int, P, P, P, c_double, ctypes.POINTER(c_double), c_double,
         c_double, P,
     ],
+    "fsx_ns2d_rk_stage_ext": [
+        P, c_int, c_int, P, P, P, c_double, ctypes.POINTER(c_double), c_double,
+        c_double, P, P,
+    ],
     "fsx_ns3d_rk_stage": [
         P, c_int, c_int, P, P, P, c_int, c_double, c_int, c_int,
         ctypes.POINTER(c_double), c_double, c_double, P,

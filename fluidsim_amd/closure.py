@@ -196,3 +196,175 @@ class CNNClosure:
                       self.step_count, _lib.dptr(wb) if wb is not None else None, s)
         self.last_grads = (grads_w, grads_b)
         return loss.value if want_loss else None
+
+    # ------------------------------------------------------------ LES use
+    def folded(self, in_scale, out_scale):
+        """Copy whose input is the raw field and output the raw subgrid term:
+        y = out_scale * net(x / in_scale) (the scales enter the first and
+        last layers, which are linear in their inputs / outputs)."""
+        torch = _torch()
+        other = CNNClosure.__new__(CNNClosure)
+        other.__dict__.update({k: v for k, v in self.__dict__.items()
+                               if k not in ("w", "b", "wb", "_work", "m1", "m2")})
+        other.w = [t.clone() for t in self.w]
+        other.b = [t.clone() for t in self.b]
+        other.w[0] /= float(in_scale)
+        other.w[-1] *= float(out_scale)
+        other.b[-1] *= float(out_scale)
+        other.wb = [torch.empty(w.shape, dtype=torch.int16, device="cuda") for w in other.w]
+        other._work = {}
+        other.m1 = other.m2 = []
+        other._refresh_bf16()
+        return other
+
+
+# ---------------------------------------------------------------- data
+def _resize(src, src_shapeK, dst, dst_shapeK, prec):
+    _lib.call("fsx_resize_fft", _lib.dptr(src), 1, src_shapeK[0], src_shapeK[1], _lib.dptr(dst),
+              1, dst_shapeK[0], dst_shapeK[1], 1, prec, _lib.stream_ptr())
+
+
+def cut_to_les(dns_oper, les_oper, a_fft):
+    """Sharp spectral cut of a DNS half spectrum to the LES grid
+    (fill_field_fft_2d, fluidsim/base/init_fields.py:300-314), then the LES
+    grid's own dealiasing: the filter whose output the LES state is."""
+    torch = _torch()
+    out = torch.zeros(tuple(les_oper.shapeK), dtype=a_fft.dtype, device="cuda")
+    a = a_fft.reshape(tuple(dns_oper.shapeK)).contiguous()
+    _resize(a, dns_oper.shapeK, out, les_oper.shapeK, dns_oper.precision)
+    les_oper.dealiasing(out)
+    return out
+
+
+def _combine(oper, out, terms):
+    """out = sum_j a_j x_j over one field (fsx_if_combine with b = e = p = 0)."""
+    n = len(terms)
+    ins = (ctypes.c_void_p * n)(*[t[0].data_ptr() for t in terms])
+    a = _lib.doubles([float(t[1]) for t in terms])
+    z = _lib.doubles([0.0] * n)
+    pw = (ctypes.c_int * n)(*([0] * n))
+    _lib.call("fsx_if_combine", oper.plan.handle, 1, _lib.dptr(out), n, ins, a, z, z, pw,
+              _lib.doubles([0.0, 0.0, 0.0]), _lib.doubles([0.0] * 4), 0, _lib.stream_ptr())
+
+
+def subgrid_terms(dns_sim, les_oper, beta=0.0):
+    """(w_bar, S) on the LES grid (real fields) from the DNS state:
+    w_bar = cut(w), S = cut(N(w)) - N(w_bar) with N the dealiased ns2d
+    nonlinear term of each grid."""
+    torch = _torch()
+    od = dns_sim.oper
+    rot = dns_sim.state.state_spect.data
+    N = torch.zeros_like(rot)
+    od.ns2d_tendency(rot, N, beta)
+    wk = cut_to_les(od, les_oper, rot[0])
+    Nk = cut_to_les(od, les_oper, N[0])
+    Nl = torch.zeros((1,) + tuple(les_oper.shapeK), dtype=rot.dtype, device="cuda")
+    wk1 = wk.reshape(Nl.shape)
+    les_oper.ns2d_tendency(wk1, Nl, beta)
+    S = torch.empty_like(Nk)
+    _combine(les_oper, S, [(Nk, 1.0), (Nl[0], -1.0)])
+    return les_oper.ifft(wk), les_oper.ifft(S)
+
+
+# ---------------------------------------------------------------- LES
+def make_les_class():
+    """The ns2d solver with the learned subgrid term added to the
+    nonlinear tendency of every Runge-Kutta stage (fsx_ns2d_rk_stage_ext)."""
+    from .solvers.ns2d.solver import Simul as SimulNS2D
+
+    class SimulLES(SimulNS2D):
+        closure_net = None  # CNNClosure with folded scales (raw w -> raw S)
+
+        def _closure_term(self, src):
+            torch = _torch()
+            oper = self.oper
+            ny, nx = oper.shapeX
+            if getattr(self, "_cl_x", None) is None:
+                self._cl_x = torch.empty((1, ny, nx, 1), dtype=torch.float32, device="cuda")
+                self._cl_y = torch.empty((1, ny, nx), dtype=torch.float32, device="cuda")
+                self._cl_k = torch.empty((1,) + tuple(oper.shapeK), dtype=src.dtype,
+                                         device="cuda")
+            x, y, k = self._cl_x, self._cl_y, self._cl_k
+            oper.ifft_as_arg(src[0], x.view(ny, nx))
+            self.closure_net.forward(x, out=y)
+            oper.fft_as_arg(y.view(ny, nx), k[0])
+            oper.dealiasing(k)
+            return k
+
+        def _engine_rk_stage(self, scheme, stage, S, tmp, t1, visc, cfl, dt_max):
+            if self.closure_net is None:
+                return super()._engine_rk_stage(scheme, stage, S, tmp, t1, visc, cfl, dt_max)
+            ext = self._closure_term(S if stage == 0 else t1)
+            _lib.call("fsx_ns2d_rk_stage_ext", self.oper.plan.handle, scheme, stage,
+                      _lib.dptr(S), _lib.dptr(tmp), _lib.dptr(t1), float(self.params.beta),
+                      visc, cfl, dt_max, _lib.dptr(ext), _lib.stream_ptr())
+
+    return SimulLES
+
+
+def ns2d_params(n, nu_2, precision="float32", forcing=True, nkmin=8, nkmax=10, rate=1.0,
+                les=False):
+    """Parameters of the forced ns2d runs of the closure workload."""
+    from .solvers.ns2d.solver import Simul
+
+    cls = make_les_class() if les else Simul
+    params = cls.create_default_params()
+    params.oper.nx = params.oper.ny = n
+    params.oper.Lx = params.oper.Ly = 2 * np.pi
+    params.oper.precision = precision
+    params.nu_2 = nu_2
+    params.time_stepping.USE_CFL = True
+    params.time_stepping.cfl_coef = 0.5
+    params.time_stepping.USE_T_END = False
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    if forcing:
+        params.forcing.enable = True
+        params.forcing.type = "tcrandom"
+        params.forcing.nkmin_forcing = nkmin
+        params.forcing.nkmax_forcing = nkmax
+        params.forcing.forcing_rate = rate
+    return cls, params
+
+
+def run_steps(sim, nsteps):
+    """Advance a Simul by nsteps (the time loop of TimeStepping.main_loop)."""
+    ts = sim.time_stepping
+    if not getattr(ts, "_prepare_main_loop_called", False):
+        ts.prepare_main_loop()
+    for _ in range(nsteps):
+        ts.one_time_step()
+
+
+def energy_spectrum(oper, rot_fft):
+    """2D (shell) energy spectrum of a vorticity field on the device (FP64)."""
+    from .base import diagnostics as dg
+
+    r = rot_fft.reshape((1,) + tuple(oper.shapeK)).contiguous()
+    return dg.run(oper, dg.NS2D_SPECTRA, dg.SHELL, [r[0]])["shell"][0]
+
+
+def train(net, samples_w, samples_s, steps, batch=16, lr=1e-3, seed=0, log=None):
+    """Adam on (w / sigma_w) -> (S / sigma_S) with random periodic shifts;
+    returns (sigma_w, sigma_S, losses)."""
+    torch = _torch()
+    W = torch.stack(samples_w).float()
+    Sx = torch.stack(samples_s).float()
+    sw = float(W.std())
+    ss = float(Sx.std())
+    W /= sw
+    Sx /= ss
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    n, H, Wd = W.shape
+    losses = []
+    for it in range(steps):
+        idx = torch.randint(0, n, (batch,), generator=g)
+        dy, dx = (int(v) for v in torch.randint(0, H, (2,), generator=g))
+        xb = torch.roll(W[idx], (dy, dx), dims=(1, 2)).unsqueeze(-1).contiguous()
+        yb = torch.roll(Sx[idx], (dy, dx), dims=(1, 2)).contiguous()
+        want = log is not None and (it % log == 0 or it == steps - 1)
+        loss = net.train_step(xb, yb, lr=lr, want_loss=want)
+        if want:
+            losses.append((it, loss))
+    return sw, ss, losses

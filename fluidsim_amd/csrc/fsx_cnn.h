@@ -64,19 +64,19 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint
 __host__ __device__ constexpr int cnn_kp(int ci) { return kCnnTaps * ci + 4; }
 __host__ __device__ constexpr int cnn_ps(int ci) { return ci + 4; }
 
-// LDS bytes of k_conv_fwd: weights [32][KP] + patch [rows + 4][W + 4][PS]
-__host__ __device__ inline size_t conv_fwd_lds(int ci, int W) {
-  return (size_t)2 * (32 * cnn_kp(ci) + (size_t)(kCnnRows + 4) * (W + 4) * cnn_ps(ci));
+// LDS bytes of k_conv_fwd: weights [32][KP] + patch [R + 4][W + 4][PS]
+__host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
+  return (size_t)2 * (32 * cnn_kp(ci) + (size_t)(R + 4) * (W + 4) * cnn_ps(ci));
 }
 
-// One workgroup (4 waves) per (image, kCnnRows output rows); each wave
-// computes 32 output channels x 32 pixels tiles.
+// One workgroup (4 waves) per (image, R output rows); each wave computes
+// 32 output channels x 32 pixels tiles.
 template <int CI>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
                                                   const uint16_t* __restrict__ w,
                                                   const float* __restrict__ bias,
                                                   float* __restrict__ y, int H, int W,
-                                                  int co_out) {
+                                                  int co_out, int R) {
   constexpr int K = kCnnTaps * CI;
   constexpr int KP = cnn_kp(CI);
   constexpr int PS = cnn_ps(CI);
@@ -84,15 +84,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x
   uint16_t* sw = cnn_sm;
   uint16_t* sp = cnn_sm + 32 * KP;
   const int WP = W + 4;
-  const int nyb = H / kCnnRows;
-  const int b = blockIdx.x / nyb, y0 = (blockIdx.x % nyb) * kCnnRows;
+  const int nyb = H / R;
+  const int b = blockIdx.x / nyb, y0 = (blockIdx.x % nyb) * R;
   for (int t = threadIdx.x; t < 32 * K / 4; t += blockDim.x) {
     const int co = (4 * t) / K, k = (4 * t) % K;
     *reinterpret_cast<uint2*>(&sw[co * KP + k]) =
         *reinterpret_cast<const uint2*>(&w[co * K + k]);
   }
   constexpr int NV = CI / 4;  // 8-byte vectors per pixel
-  const int nrow = kCnnRows + 4;
+  const int nrow = R + 4;
   for (int t = threadIdx.x; t < nrow * WP * NV; t += blockDim.x) {
     const int v = t % NV, rest = t / NV;
     const int c = rest % WP, r = rest / WP;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int m = lane & 31, g = lane >> 5;
   const int nchunk = W / 32;
-  for (int tile = wid; tile < kCnnRows * nchunk; tile += nw) {
+  for (int tile = wid; tile < R * nchunk; tile += nw) {
     const int r = tile / nchunk, n0 = (tile % nchunk) * 32;
     f32x16 acc = {};
 #pragma unroll 4

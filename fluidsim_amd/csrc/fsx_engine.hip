@@ -754,6 +754,7 @@ RKArgs<T> make_rk(fsx_plan_s* p, int scheme, int stage, int nvar, void* S,
   a.fnyc = p->fnyc;
   a.fnzc = p->fnzc;
   a.fnkxc = p->fnkxc;
+  for (int v = 0; v < 4; ++v) a.ext[v] = nullptr;
   return a;
 }
 
@@ -857,7 +858,7 @@ int cfl_end(fsx_plan_s* p, int ncomp, double cfl, double dt_max, hipStream_t s) 
 template <typename T>
 int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
                void* t1, double beta, const double* visc, double cfl,
-               double dt_max, hipStream_t s) {
+               double dt_max, hipStream_t s, const void* ext = nullptr) {
   const bool use_cfl = cfl > 0 && stage == 0;
   int rc = 0;
   if (use_cfl && (rc = cfl_begin(p, s))) return rc;
@@ -874,6 +875,7 @@ int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
   op.ym = g.ym;
   op.mask = p->d_mask;
   op.rk = make_rk<T>(p, scheme, stage, 1, S, tmp, t1, nullptr, 0, 0, visc);
+  op.rk.ext[0] = reinterpret_cast<const cpx<T>*>(ext);
   note(K_ZFWD, (double)g.mx * p->ny * sizeof(cpx<T>) +
                    rk_bytes(p, scheme, stage, 1, sizeof(cpx<T>)));
   return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s, true);
@@ -1292,6 +1294,22 @@ int fsx_ns2d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
                                           cfl, dt_max, s)
                       : do_ns2d_rk<double>(p, scheme, stage, S, tmp, t1, beta,
                                            visc, cfl, dt_max, s);
+}
+
+int fsx_ns2d_rk_stage_ext(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                          void* t1, double beta, const double* visc, double cfl,
+                          double dt_max, const void* ext_fft, void* stream) {
+  PlanScope scope_(p);
+  if (!p || !S || !t1 || !visc) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 2 || p->nf_max < 4) return fail(FSX_ERR_ARG, "ns2d needs a 2D plan with nf_max >= 4");
+  if (scheme != 4 && scheme != 2) return fail(FSX_ERR_ARG, "scheme must be 2 or 4");
+  if (stage < 0 || stage >= scheme) return fail(FSX_ERR_ARG, "bad stage");
+  if (scheme == 4 && !tmp) return fail(FSX_ERR_ARG, "RK4 needs tmp");
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == 4 ? do_ns2d_rk<float>(p, scheme, stage, S, tmp, t1, beta, visc,
+                                          cfl, dt_max, s, ext_fft)
+                      : do_ns2d_rk<double>(p, scheme, stage, S, tmp, t1, beta,
+                                           visc, cfl, dt_max, s, ext_fft);
 }
 
 int fsx_ns3d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
@@ -2144,20 +2162,24 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   if (co_out < 1 || co_out > 32) return fail(FSX_ERR_ARG, "output channels must be 1..32");
   if (H < kCnnRows || H % kCnnRows || W < 32 || W % 32)
     return fail(FSX_ERR_ARG, "H must be a multiple of 4 and W a multiple of 32");
-  const size_t lds = conv_fwd_lds(ci, W);
+  // rows per workgroup: 4 when there are enough workgroups, fewer for
+  // small batches (more workgroups, smaller patches)
+  int R = kCnnRows;
+  while (R > 1 && (long long)B * (H / R) < 512) R >>= 1;
+  const size_t lds = conv_fwd_lds(ci, W, R);
   if (lds > kLdsMax) return fail(FSX_ERR_ARG, "image too wide for the convolution tile");
-  const unsigned nblk = (unsigned)(B * (H / kCnnRows));
+  const unsigned nblk = (unsigned)(B * (H / R));
   const auto* xi = reinterpret_cast<const uint16_t*>(x);
   const auto* wi = reinterpret_cast<const uint16_t*>(w);
   int rc;
   if (ci == 8) {
-    if ((rc = set_lds(k_conv_fwd<8>, lds))) return rc;
+    if ((rc = set_lds(k_conv_fwd<8>, conv_fwd_lds(8, W, kCnnRows)))) return rc;
     hipLaunchKernelGGL(k_conv_fwd<8>, dim3(nblk), dim3(256), lds, s, xi, wi, bias, y, H, W,
-                       co_out);
+                       co_out, R);
   } else {
-    if ((rc = set_lds(k_conv_fwd<32>, lds))) return rc;
+    if ((rc = set_lds(k_conv_fwd<32>, conv_fwd_lds(32, W, kCnnRows)))) return rc;
     hipLaunchKernelGGL(k_conv_fwd<32>, dim3(nblk), dim3(256), lds, s, xi, wi, bias, y, H, W,
-                       co_out);
+                       co_out, R);
   }
   FSX_HIP(hipGetLastError());
   return 0;

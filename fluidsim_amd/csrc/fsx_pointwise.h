@@ -114,6 +114,8 @@ struct RKArgs {
   const cpx<T>* fc;
   long long fvs;
   int fnyc, fnzc, fnkxc;
+  // full-size additive tendencies (null: none), added with the forcing
+  const cpx<T>* ext[4];
 };
 
 // Index in the coarse forcing box of the mode (ikx, iky, ikz) (signed ky,
@@ -151,6 +153,9 @@ __device__ __forceinline__ void rk_point(const RKArgs<T>& a, const Visc& visc, d
       for (int v = 0; v < NV; ++v) N[v] = N[v] + a.fc[v * a.fvs + c];
     }
   }
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+    if (a.ext[v]) N[v] = N[v] + a.ext[v][idx];
   if constexpr (NV >= 3) {
     if (a.project != PROJ_NONE) {
       project3(N[0], N[1], N[2], a.project, ikx, iky, ikz, bx);

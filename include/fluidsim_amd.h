@@ -117,6 +117,14 @@ int fsx_ns2d_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp,
                       void* t1, double beta, const double* visc, double cfl,
                       double dt_max, void* stream);
 
+/* fsx_ns2d_rk_stage with a full-size additive tendency ext_fft (one K
+ * field of the plan, NULL: none) added to N before the stage algebra, like
+ * the forcing (fluidsim/solvers/ns2d/solver.py:190-191): the hook of the
+ * learned LES closure (no counterpart in the reference). */
+int fsx_ns2d_rk_stage_ext(fsx_plan p, int scheme, int stage, void* S, void* tmp,
+                          void* t1, double beta, const double* visc, double cfl,
+                          double dt_max, const void* ext_fft, void* stream);
+
 /* Fused Runge-Kutta stage of ns3d (3 velocity fields), same contract as
  * fsx_ns2d_rk_stage with the tendency of fluidsim/solvers/ns3d/solver.py
  * :180-253 and the projection of fsx_rk_stage. */

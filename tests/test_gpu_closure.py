@@ -1,0 +1,142 @@
+"""GPU tests of the learned-closure LES pipeline (fluidsim_amd.closure).
+
+* subgrid terms (cut of the DNS tendency minus the LES tendency of the cut
+  state) against the FP64 oracle (ns2d tendencies of oracle/solvers.py,
+  fill_field_fft_2d of oracle/resize.py): 1e-10;
+* the additive-tendency hook of the fused stages (fsx_ns2d_rk_stage_ext)
+  against the oracle with the same constant term added to its tendencies
+  (as fluidsim adds the forcing, ns2d/solver.py:190-191): 1e-12 (FP64);
+* an LES whose network outputs zero reproduces the plain solver bit for
+  bit; a small pipeline (forced DNS -> samples -> training -> LES) runs.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import resize as R
+from oracle.operators import Oper2D
+from oracle.solvers import NS2D
+from fluidsim_amd import closure as CL
+from fluidsim_amd.synthetic import ns2d_noise_rot_fft
+
+from .gpu_helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_subgrid_terms_vs_oracle():
+    n, m = 128, 32
+    Cls, params = CL.ns2d_params(n, 1e-3, precision="float64", forcing=False)
+    dns = Cls(params)
+    rot0 = ns2d_noise_rot_fft(n, n, seed=3, kmax=40)
+    dns.state.init_statespect_from(rot_fft=rot0)
+    _, pl = CL.ns2d_params(m, 1e-3, precision="float64", forcing=False)
+    les_oper = Cls(pl).oper
+    w_bar, S = CL.subgrid_terms(dns, les_oper)
+
+    od, ol = Oper2D(n, n), Oper2D(m, m)
+    rot = dns.state.get_var_numpy("rot_fft")
+    N = NS2D(od).tendencies_nonlin(np.array([rot]))[0]
+
+    def cut(a):
+        out = np.zeros(ol.shapeK, complex)
+        R.fill_field_fft_2d(a, out)
+        ol.dealiasing(out)
+        return out
+
+    wk, Nk = cut(rot), cut(N)
+    Nl = NS2D(ol).tendencies_nonlin(np.array([wk]))[0]
+    ref_w, ref_S = ol.ifft(wk), ol.ifft(Nk - Nl)
+    ew, es = rel_l2(w_bar.cpu().numpy(), ref_w), rel_l2(S.cpu().numpy(), ref_S)
+    print(f"w_bar rel L2 = {ew:.2e}, S rel L2 = {es:.2e}")
+    assert ew < 1e-10 and es < 1e-10
+    assert np.abs(ref_S).max() > 1e-3 * np.abs(ol.ifft(Nk)).max()  # S is not trivial
+
+
+class _FixedTerm:
+    """Stands for a network: the hook adds a fixed K field at every stage."""
+
+
+def test_ext_hook_vs_oracle_constant_term():
+    n, nsteps, dt = 64, 3, 0.01
+    SimulLES = CL.make_les_class()
+    _, params = CL.ns2d_params(n, 1e-3, precision="float64", forcing=False)
+    params.time_stepping.USE_CFL = False
+    params.time_stepping.deltat0 = dt
+    sim = SimulLES(params)
+    rot0 = ns2d_noise_rot_fft(n, n, seed=4, kmax=16)
+    sim.state.init_statespect_from(rot_fft=rot0)
+    oper = Oper2D(n, n)
+    rng = np.random.default_rng(0)
+    F = oper.fft(rng.standard_normal(oper.shapeX))
+    oper.dealiasing(F)
+    Fd = torch.from_numpy(np.array([F])).cuda()
+    sim.closure_net = _FixedTerm()
+    sim._closure_term = lambda src: Fd
+    S0 = sim.state.get_var_numpy("rot_fft")
+    CL.run_steps(sim, nsteps)
+    got = sim.state.get_var_numpy("rot_fft")
+
+    class Constant:
+        def compute(self, state_spect, t, dt):
+            return np.array([F])
+
+    orc = NS2D(oper, nu_2=1e-3, deltat0=dt)
+    orc.state_spect[0] = S0
+    orc.forcing = Constant()
+    orc.run(nsteps)
+    err = rel_l2(got, orc.state_spect[0])
+    print(f"ext hook rel L2 = {err:.2e}")
+    assert err <= 1e-12
+
+
+def test_zero_network_reproduces_plain_solver():
+    n, nsteps = 64, 5
+    SimulLES = CL.make_les_class()
+    rot0 = ns2d_noise_rot_fft(n, n, seed=5, kmax=16)
+    outs = []
+    for les in (False, True):
+        Cls, params = CL.ns2d_params(n, 1e-3, forcing=False, les=True)
+        sim = Cls(params)
+        sim.state.init_statespect_from(rot_fft=rot0)
+        if les:
+            net = CL.CNNClosure(n_in=1, nlayers=3, seed=0)
+            for w in net.w + net.b:
+                w.zero_()
+            net._refresh_bf16()
+            sim.closure_net = net
+        CL.run_steps(sim, nsteps)
+        outs.append(sim.state.get_var_numpy("rot_fft"))
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_small_pipeline_runs():
+    """forced DNS 128^2 -> 16 samples on 32^2 -> 40 training steps -> LES 32^2."""
+    n, m = 128, 32
+    Cls, params = CL.ns2d_params(n, 1e-3, rate=0.5, nkmin=3, nkmax=5)
+    np.random.seed(1)
+    dns = Cls(params)
+    dns.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(n, n, seed=6, kmax=20))
+    _, pl = CL.ns2d_params(m, 1e-3, forcing=False)
+    les_oper = Cls(pl).oper
+    CL.run_steps(dns, 20)
+    ws, ss = [], []
+    for _ in range(16):
+        CL.run_steps(dns, 3)
+        w, s = CL.subgrid_terms(dns, les_oper)
+        ws.append(w)
+        ss.append(s)
+    net = CL.CNNClosure(n_in=1, nlayers=6, seed=1)
+    sw, s_s, losses = CL.train(net, ws, ss, steps=40, batch=8, lr=2e-3, log=10)
+    print("losses", losses)
+    assert losses[-1][1] < losses[0][1]
+    SimulLES, lp = CL.ns2d_params(m, 1e-3, rate=0.5, nkmin=3, nkmax=5, les=True)
+    np.random.seed(2)
+    les = SimulLES(lp)
+    wk = CL.cut_to_les(dns.oper, les_oper, dns.state.state_spect.data[0])
+    les.state.init_statespect_from(rot_fft=wk.cpu().numpy())
+    les.closure_net = net.folded(sw, s_s)
+    CL.run_steps(les, 20)
+    E = CL.energy_spectrum(les.oper, les.state.state_spect.data)
+    assert np.all(np.isfinite(E)) and E.sum() > 0
