@@ -13,6 +13,13 @@ runs: 256 at 128^2, 192 at 256^2, 64 at 512^2, seed = run index, nu_2
 log-uniform in [1e-4, 1e-2], RK4 at each run's fixed initial-CFL dt, one
 HIP graph launch per ensemble step); the value is ensemble steps/s.
 
+``--closure``: the learned-closure workload instead: forced ns2d DNS
+(tcrandom forcing at k in [8, 10], nu_2 = 1e-5) generates filtered
+training pairs, the convolutional closure is trained on the device, then a
+held-out DNS is compared with the coarse LES with and without the closure
+over the same simulated time; the value is the DNS / LES wall-time ratio,
+with the time-averaged energy-spectrum errors against the filtered DNS.
+
 Prints ONE JSON line with the metric, a ``roofline`` object for the
 dominant kernel (algorithmic bytes per launch / average launch duration,
 measured with HIP events over the timed region), the step-level roofline of
@@ -308,6 +315,173 @@ def bench_ensemble(args, torch, _lib, copy_gbps):
     print(json.dumps(result), flush=True)
 
 
+def bench_closure(args, torch, _lib):
+    from fluidsim_amd import closure as CL
+    from fluidsim_amd.synthetic import ns2d_noise_rot_fft
+
+    nd, nl, nu = args.closure_dns, args.closure_les, 1e-5
+
+    def log(msg):
+        print(f"[closure] {msg}", file=sys.stderr, flush=True)
+
+    def new_dns(seed):
+        Cls, p = CL.ns2d_params(nd, nu)
+        np.random.seed(seed)
+        with contextlib.redirect_stdout(sys.stderr):
+            sim = Cls(p)
+        sim.state.init_statespect_from(
+            rot_fft=ns2d_noise_rot_fft(nd, nd, seed=seed, kmax=12, velo_max=0.5))
+        return sim
+
+    def run_until(sim, t):
+        ts = sim.time_stepping
+        if not getattr(ts, "_prepare_main_loop_called", False):
+            with contextlib.redirect_stdout(sys.stderr):
+                ts.prepare_main_loop()
+        n = 0
+        while ts.t < t - 1e-12:
+            ts.one_time_step()
+            n += 1
+        return n
+
+    Cls, pl = CL.ns2d_params(nl, nu, forcing=False)
+    with contextlib.redirect_stdout(sys.stderr):
+        les_oper = Cls(pl).oper
+    # 1. training pairs from forced DNS runs
+    t0 = time.perf_counter()
+    ws, ss = [], []
+    for r in range(args.closure_runs):
+        dns = new_dns(100 + r)
+        run_until(dns, args.closure_spinup)
+        for k in range(args.closure_snaps):
+            run_until(dns, args.closure_spinup + (k + 1) * args.closure_snap_dt)
+            w, sgs = CL.subgrid_terms(dns, les_oper)
+            ws.append(w.float())
+            ss.append(sgs.float())
+        log(f"run {r}: t = {dns.time_stepping.t:.2f}, it = {dns.time_stepping.it}")
+        del dns
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    # 2. training on the device
+    net = CL.CNNClosure(n_in=1, nlayers=6, seed=0)
+    t0 = time.perf_counter()
+    half = args.closure_train_steps // 2
+    sw, s_s, l1 = CL.train(net, ws, ss, half, batch=32, lr=1e-3, seed=0, log=100)
+    _, _, l2 = CL.train(net, ws, ss, args.closure_train_steps - half, batch=32, lr=3e-4,
+                        seed=1, log=100)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"training losses {l1[0]} ... {l2[-1]}")
+    del ws, ss
+    # 3. held-out DNS against the coarse LES with and without the closure
+    dns = new_dns(999)
+    run_until(dns, args.closure_spinup)
+    wk = CL.cut_to_les(dns.oper, les_oper, dns.state.state_spect.data[0]).cpu().numpy()
+    runs = {}
+    for name in ("les_closure", "les_plain"):
+        SimulLES, lp = CL.ns2d_params(nl, nu, les=True)
+        np.random.seed(7)
+        with contextlib.redirect_stdout(sys.stderr):
+            les = SimulLES(lp)
+        les.state.init_statespect_from(rot_fft=wk)
+        les.time_stepping.t = dns.time_stepping.t
+        if name == "les_closure":
+            les.closure_net = net.folded(sw, s_s)
+        runs[name] = les
+    runs["dns"] = dns
+    # same forcing in the three runs: the coarse forcing box does not depend
+    # on the grid, so the LES makers take the DNS maker's random pair and
+    # renewal time, and each run draws its renewals from its own copy of
+    # the random state
+    src = dns.forcing.forcing_maker
+    for name in ("les_closure", "les_plain"):
+        mk = runs[name].forcing.forcing_maker
+        if mk.forcing0.shape != src.forcing0.shape:
+            raise ValueError("DNS and LES coarse forcing boxes differ")
+        mk.t_last_change = src.t_last_change
+        mk._seed0, mk._seed1 = src._seed0, src._seed1
+        mk.forcing0, mk.forcing1 = src.forcing0.copy(), src.forcing1.copy()
+        mk._upload_pair(mk.forcing0, mk.forcing1)
+    rng_state = {k: np.random.get_state() for k in runs}
+    spectra = {k: [] for k in runs}
+    wall = {k: 0.0 for k in runs}
+    steps = {k: 0 for k in runs}
+    t_start = dns.time_stepping.t
+    nsamp = int(round(args.closure_eval_t / args.closure_sample_dt))
+    for j in range(nsamp):
+        t_next = t_start + (j + 1) * args.closure_sample_dt
+        for k, sim in runs.items():
+            np.random.set_state(rng_state[k])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            steps[k] += run_until(sim, t_next)
+            torch.cuda.synchronize()
+            wall[k] += time.perf_counter() - t0
+            rng_state[k] = np.random.get_state()
+            rot = sim.state.state_spect.data
+            if k == "dns":
+                rot = CL.cut_to_les(sim.oper, les_oper, rot[0])
+            spectra[k].append(CL.energy_spectrum(les_oper, rot))
+        for k, sim in runs.items():
+            if not np.isfinite(spectra[k][-1]).all():
+                raise ValueError(f"{k}: non-finite spectrum at t = {sim.time_stepping.t}")
+    kmax = int(nl / 3)  # kept shells of the LES grid (2/3 truncation)
+    mean = {k: np.mean(v, axis=0)[1: kmax + 1] for k, v in spectra.items()}
+
+    def err(k, lo=0):
+        d = mean[k][lo:] - mean["dns"][lo:]
+        return float(np.linalg.norm(d) / np.linalg.norm(mean["dns"][lo:]))
+
+    def log_err(k):
+        return float(np.mean(np.abs(np.log(mean[k] / mean["dns"]))))
+
+    speedup = wall["dns"] / wall["les_closure"]
+    result = {
+        "metric": "closure-LES vs DNS speedup at equal error",
+        "value": speedup,
+        "unit": "DNS wall time / LES wall time",
+        "n_gpus": 1,
+        "steps": steps["les_closure"],
+        "warmup": 0,
+        "ms_per_step": wall["les_closure"] / max(steps["les_closure"], 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 (network bf16 x bf16 -> f32)",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"ns2d forced (tcrandom k in [8, 10], rate 1), nu_2 = 1e-5; DNS "
+                         f"{nd}^2 FP32; LES {nl}^2 FP32 with a 6-layer 5x5 32-channel GELU "
+                         f"network (BF16 weights, FP32 accumulation) trained on "
+                         f"{args.closure_runs} DNS runs x {args.closure_snaps} filtered "
+                         f"snapshots; evaluation over {args.closure_eval_t} time units "
+                         f"from a held-out DNS state"),
+            "parallelism": "single GPU",
+        },
+        "spectrum_error": {
+            "les_closure": err("les_closure"),
+            "les_plain": err("les_plain"),
+            "shells": [1, kmax],
+            "definition": "relative L2 of the time-averaged energy spectrum vs the "
+                          "filtered DNS",
+            "small_scales": {
+                "shells": [11, kmax],
+                "les_closure": err("les_closure", 10),
+                "les_plain": err("les_plain", 10),
+            },
+            "mean_abs_log_ratio": {
+                "les_closure": log_err("les_closure"),
+                "les_plain": log_err("les_plain"),
+            },
+        },
+        "wall_s": wall,
+        "steps_by_run": steps,
+        "timings_s": {"training_data": t_data, "training": t_train},
+        "training_loss": {"first": l1[0][1], "last": l2[-1][1]},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def read_traffic(kernel_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -328,6 +502,17 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--ensemble", action="store_true",
                     help="time the batched ns2d ensemble (512 runs) instead")
+    ap.add_argument("--closure", action="store_true",
+                    help="run the learned-closure LES workload instead")
+    ap.add_argument("--closure-dns", type=int, default=2048)
+    ap.add_argument("--closure-les", type=int, default=128)
+    ap.add_argument("--closure-runs", type=int, default=8)
+    ap.add_argument("--closure-snaps", type=int, default=100)
+    ap.add_argument("--closure-snap-dt", type=float, default=0.1)
+    ap.add_argument("--closure-spinup", type=float, default=3.0)
+    ap.add_argument("--closure-train-steps", type=int, default=2000)
+    ap.add_argument("--closure-eval-t", type=float, default=4.0)
+    ap.add_argument("--closure-sample-dt", type=float, default=0.2)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first "
                          "(calibration kernel for the HBM byte counters)")
@@ -350,6 +535,8 @@ def main():
     copy_gbps = measured_copy_bandwidth(torch)
     if args.ensemble:
         return bench_ensemble(args, torch, _lib, copy_gbps)
+    if args.closure:
+        return bench_closure(args, torch, _lib)
     if args.copy_mb:
         src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
