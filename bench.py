@@ -425,6 +425,28 @@ def bench_closure(args, torch, _lib):
         for k, sim in runs.items():
             if not np.isfinite(spectra[k][-1]).all():
                 raise ValueError(f"{k}: non-finite spectrum at t = {sim.time_stepping.t}")
+    # per-launch rate of the hidden-layer convolutions (HIP events on the
+    # LES stream) over extra LES steps after the timed evaluation
+    les = runs["les_closure"]
+    plan = les.oper.plan.handle
+    c = _lib.ctypes
+    ms, fl, cnt = c.c_double(), c.c_double(), c.c_longlong()
+    _lib.call("fsx_timing_enable", plan, 1)
+    _lib.call("fsx_timing_read_network", plan, ms, fl, cnt, 1)
+    np.random.set_state(rng_state["les_closure"])
+    for _ in range(20):
+        les.time_stepping.one_time_step()
+    _lib.call("fsx_timing_read_network", plan, ms, fl, cnt, 1)
+    _lib.call("fsx_timing_enable", plan, 0)
+    tflops = fl.value / (ms.value * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+        "frac": tflops / 2500.0, "traffic": None,
+        "kernel": f"k_conv_fwd<32, 1>: hidden 32 -> 32 layer on one {nl}^2 field",
+        "launch_us": ms.value / max(cnt.value, 1) * 1e3,
+        "flops_per_launch": fl.value / max(cnt.value, 1),
+        "launches": cnt.value,
+    }
     kmax = int(nl / 3)  # kept shells of the LES grid (2/3 truncation)
     mean = {k: np.mean(v, axis=0)[1: kmax + 1] for k, v in spectra.items()}
 
@@ -474,8 +496,10 @@ def bench_closure(args, torch, _lib):
                 "les_plain": log_err("les_plain"),
             },
         },
+        "roofline": roofline,
         "wall_s": wall,
         "steps_by_run": steps,
+        "ms_per_step_by_run": {k: wall[k] / max(steps[k], 1) * 1e3 for k in wall},
         "timings_s": {"training_data": t_data, "training": t_train},
         "training_loss": {"first": l1[0][1], "last": l2[-1][1]},
     }

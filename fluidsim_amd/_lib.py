@@ -105,7 +105,9 @@ SIGNATURES = {
     "fsx_graph_launch": [P, P],
     "fsx_graph_destroy": [P],
     "fsx_cnn_probe": [P, P, P, P],
+    "fsx_cnn_probe16": [P, P, P, P],
     "fsx_cnn_conv_fwd": [P, c_int, c_int, c_int, c_int, P, P, c_int, P, P],
+    "fsx_cnn_conv_fwd_act": [P, c_int, c_int, c_int, c_int, P, P, P, P, P],
     "fsx_cnn_conv_flip": [P, c_int, c_int, c_int, P, P],
     "fsx_cnn_wgrad_workspace": [c_int, c_int, c_int, ctypes.POINTER(ctypes.c_longlong)],
     "fsx_cnn_conv_wgrad": [
@@ -122,7 +124,16 @@ SIGNATURES = {
         c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), P, P, P, c_int, c_int, c_int,
         P, P, ctypes.c_longlong, P,
     ],
+    "fsx_ns2d_closure_workspace": [P, ctypes.POINTER(ctypes.c_longlong)],
+    "fsx_ns2d_closure_term": [
+        P, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), P, P, P, P, P,
+        ctypes.c_longlong, P,
+    ],
     "fsx_timing_enable": [P, c_int],
+    "fsx_timing_read_network": [
+        P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
+        ctypes.POINTER(ctypes.c_longlong), c_int,
+    ],
     "fsx_timing_read": [
         P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
         ctypes.POINTER(ctypes.c_longlong), c_int,

@@ -103,8 +103,7 @@ class CNNClosure:
             npix = B * H * W
             self._work[key] = {
                 "x": torch.empty((B, H, W, 8), dtype=torch.int16, device="cuda"),
-                "fwd": torch.empty(2 * npix * 32 * 2 + npix * 32 * 4, dtype=torch.uint8,
-                                   device="cuda"),
+                "fwd": torch.empty(2 * npix * 32 * 2, dtype=torch.uint8, device="cuda"),
             }
         return self._work[key]
 
@@ -126,14 +125,24 @@ class CNNClosure:
         work = self._buffers(B, H, W)["fwd"]
         if out is None:
             out = torch.empty((B, H, W), dtype=torch.float32, device="cuda")
-        nl = self.nlayers
-        cin = (ctypes.c_int * nl)(*self.cin)
-        cout = (ctypes.c_int * nl)(*self.cout)
-        wp = (ctypes.c_void_p * nl)(*[t.data_ptr() for t in self.wb])
-        bp = (ctypes.c_void_p * nl)(*[t.data_ptr() for t in self.b])
+        nl, cin, cout, wp, bp = self.layer_args()
         _lib.call("fsx_cnn_forward", nl, cin, cout, wp, bp, _lib.dptr(xb), B, H, W,
                   _lib.dptr(out), _lib.dptr(work), work.numel(), _lib.stream_ptr())
         return out
+
+    def layer_args(self):
+        """(nl, cin, cout, BF16 weight pointers, bias pointers) as ctypes
+        arrays; the tensors are updated in place, so the pointers hold."""
+        if getattr(self, "_layer_args", None) is None:
+            nl = self.nlayers
+            self._layer_args = (
+                nl,
+                (ctypes.c_int * nl)(*self.cin),
+                (ctypes.c_int * nl)(*self.cout),
+                (ctypes.c_void_p * nl)(*[t.data_ptr() for t in self.wb]),
+                (ctypes.c_void_p * nl)(*[t.data_ptr() for t in self.b]),
+            )
+        return self._layer_args
 
     # ------------------------------------------------------------ training
     def train_step(self, x, target, lr=None, want_loss=True):
@@ -148,14 +157,19 @@ class CNNClosure:
         for l in range(self.nlayers):
             last = l == self.nlayers - 1
             y = torch.empty((B, H, W, self.cout[l]), **dev)
-            _lib.call("fsx_cnn_conv_fwd", _lib.dptr(acts[l]), B, H, W, self.cin[l],
-                      _lib.dptr(self.wb[l]), _lib.dptr(self.b[l]), self.cout[l],
-                      _lib.dptr(y), s)
-            pres.append(y)
-            if not last:
+            if last:
+                _lib.call("fsx_cnn_conv_fwd", _lib.dptr(acts[l]), B, H, W, self.cin[l],
+                          _lib.dptr(self.wb[l]), _lib.dptr(self.b[l]), self.cout[l],
+                          _lib.dptr(y), s)
+            else:
+                # pre-activations (for the backward pass) and the BF16 GELU
+                # activations from one launch
                 a = torch.empty((B, H, W, 32), dtype=torch.int16, device="cuda")
-                _lib.call("fsx_cnn_act", _lib.dptr(y), npix, 32, 32, 1, _lib.dptr(a), s)
+                _lib.call("fsx_cnn_conv_fwd_act", _lib.dptr(acts[l]), B, H, W, self.cin[l],
+                          _lib.dptr(self.wb[l]), _lib.dptr(self.b[l]), _lib.dptr(y),
+                          _lib.dptr(a), s)
                 acts.append(a)
+            pres.append(y)
         # loss gradient w.r.t. the output, BF16 with 8 channels
         d = torch.empty((B, H, W, 8), dtype=torch.int16, device="cuda")
         part = torch.empty(256, dtype=torch.float64, device="cuda")
@@ -205,7 +219,7 @@ class CNNClosure:
         torch = _torch()
         other = CNNClosure.__new__(CNNClosure)
         other.__dict__.update({k: v for k, v in self.__dict__.items()
-                               if k not in ("w", "b", "wb", "_work", "m1", "m2")})
+                               if k not in ("w", "b", "wb", "_work", "m1", "m2", "_layer_args")})
         other.w = [t.clone() for t in self.w]
         other.b = [t.clone() for t in self.b]
         other.w[0] /= float(in_scale)
@@ -276,20 +290,24 @@ def make_les_class():
         closure_net = None  # CNNClosure with folded scales (raw w -> raw S)
 
         def _closure_term(self, src):
+            """Network term in spectral space from the vorticity src[0]: one
+            engine call (c2r, network, r2c, dealiasing)."""
             torch = _torch()
-            oper = self.oper
-            ny, nx = oper.shapeX
-            if getattr(self, "_cl_x", None) is None:
-                self._cl_x = torch.empty((1, ny, nx, 1), dtype=torch.float32, device="cuda")
-                self._cl_y = torch.empty((1, ny, nx), dtype=torch.float32, device="cuda")
-                self._cl_k = torch.empty((1,) + tuple(oper.shapeK), dtype=src.dtype,
+            net = self.closure_net
+            if net.n_in != 1:
+                raise ValueError("the LES closure network takes the vorticity only")
+            plan = self.oper.plan.handle
+            if getattr(self, "_cl_k", None) is None:
+                nb = ctypes.c_longlong(0)
+                _lib.call("fsx_ns2d_closure_workspace", plan, nb)
+                self._cl_work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+                self._cl_k = torch.empty((1,) + tuple(self.oper.shapeK), dtype=src.dtype,
                                          device="cuda")
-            x, y, k = self._cl_x, self._cl_y, self._cl_k
-            oper.ifft_as_arg(src[0], x.view(ny, nx))
-            self.closure_net.forward(x, out=y)
-            oper.fft_as_arg(y.view(ny, nx), k[0])
-            oper.dealiasing(k)
-            return k
+            nl, cin, cout, wp, bp = net.layer_args()
+            _lib.call("fsx_ns2d_closure_term", plan, nl, cin, cout, wp, bp, _lib.dptr(src),
+                      _lib.dptr(self._cl_k), _lib.dptr(self._cl_work), self._cl_work.numel(),
+                      _lib.stream_ptr())
+            return self._cl_k
 
         def _engine_rk_stage(self, scheme, stage, S, tmp, t1, visc, cfl, dt_max):
             if self.closure_net is None:
@@ -345,9 +363,15 @@ def energy_spectrum(oper, rot_fft):
     return dg.run(oper, dg.NS2D_SPECTRA, dg.SHELL, [r[0]])["shell"][0]
 
 
-def train(net, samples_w, samples_s, steps, batch=16, lr=1e-3, seed=0, log=None):
+def train(net, samples_w, samples_s, steps, batch=16, lr=1e-3, seed=0, log=None,
+          symmetries=True):
     """Adam on (w / sigma_w) -> (S / sigma_S) with random periodic shifts;
-    returns (sigma_w, sigma_S, losses)."""
+    returns (sigma_w, sigma_S, losses).
+
+    With ``symmetries``, each batch is also turned by a random multiple of
+    90 degrees and mirrored at random: the 2D Navier-Stokes tendency commutes
+    with the rotations of a square periodic box, and a mirror flips the sign
+    of the vorticity and of its tendency alike (w is a pseudoscalar)."""
     torch = _torch()
     W = torch.stack(samples_w).float()
     Sx = torch.stack(samples_s).float()
@@ -361,8 +385,16 @@ def train(net, samples_w, samples_s, steps, batch=16, lr=1e-3, seed=0, log=None)
     for it in range(steps):
         idx = torch.randint(0, n, (batch,), generator=g)
         dy, dx = (int(v) for v in torch.randint(0, H, (2,), generator=g))
-        xb = torch.roll(W[idx], (dy, dx), dims=(1, 2)).unsqueeze(-1).contiguous()
-        yb = torch.roll(Sx[idx], (dy, dx), dims=(1, 2)).contiguous()
+        xb = torch.roll(W[idx], (dy, dx), dims=(1, 2))
+        yb = torch.roll(Sx[idx], (dy, dx), dims=(1, 2))
+        if symmetries and H == Wd:
+            turn, mirror = (int(v) for v in torch.randint(0, 4, (2,), generator=g))
+            if turn:
+                xb, yb = torch.rot90(xb, turn, (1, 2)), torch.rot90(yb, turn, (1, 2))
+            if mirror % 2:
+                xb, yb = -torch.flip(xb, (2,)), -torch.flip(yb, (2,))
+        xb = xb.unsqueeze(-1).contiguous()
+        yb = yb.contiguous()
         want = log is not None and (it % log == 0 or it == steps - 1)
         loss = net.train_step(xb, yb, lr=lr, want_loss=want)
         if want:

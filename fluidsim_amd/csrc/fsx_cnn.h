@@ -2,8 +2,8 @@
 //
 // Network: 5x5 periodic convolutions (the domain is periodic), 32 channels,
 // GELU between layers; BF16 weights and activations, FP32 accumulation on
-// MFMA (v_mfma_f32_32x32x8_bf16), FP32 pre-activations, FP32 master
-// weights with Adam.
+// MFMA (v_mfma_f32_32x32x16_bf16 in the convolutions, the 32x32x8 form in
+// the weight gradients), FP32 pre-activations, FP32 master weights with Adam.
 //
 // Layouts (channels last):
 //   activations  x[B][H][W][C]  BF16, C = 8 or 32 (channel-padded with 0)
@@ -29,6 +29,7 @@ namespace fsx {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16-byte moves
 
 constexpr int kCnnTaps = 25;
 constexpr int kCnnRows = 4;  // output rows per workgroup of the convolutions
@@ -40,6 +41,16 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) {
   return __uint_as_float((uint32_t)h << 16);
+}
+
+// GELU, exact (erf) form
+__device__ __forceinline__ float gelu(float v) {
+  return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float v) {
+  const float cdf = 0.5f * (1.0f + erff(v * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * v * v);
+  return cdf + v * pdf;
 }
 
 __device__ __forceinline__ f32x16 mfma_bf16(bf16x4 a, bf16x4 b, f32x16 c) {
@@ -61,8 +72,39 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint
   for (int i = 0; i < 16; ++i) D[(8 * (i >> 2) + 4 * g + (i & 3)) * 32 + m] = acc[i];
 }
 
-__host__ __device__ constexpr int cnn_kp(int ci) { return kCnnTaps * ci + 4; }
-__host__ __device__ constexpr int cnn_ps(int ci) { return ci + 4; }
+// v_mfma_f32_32x32x16_bf16 (gfx950): lane supplies A[m][8g .. 8g+7] and
+// B[8g .. 8g+7][m]; the result layout is the one of the 32x32x8 form.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x16 mfma16_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// D (32 x 32, row-major FP32) = A (32 x 16 BF16, row-major) x B (16 x 32
+// BF16, row-major): one wave; checks the operand layout of the K = 16 form.
+__global__ __launch_bounds__(64) void k_mfma16_probe(const uint16_t* A, const uint16_t* B,
+                                                     float* D) {
+  const int lane = threadIdx.x, m = lane & 31, g = lane >> 5;
+  uint16_t ha[8], hb[8];
+  for (int j = 0; j < 8; ++j) {
+    ha[j] = A[m * 16 + 8 * g + j];
+    hb[j] = B[(8 * g + j) * 32 + m];
+  }
+  bf16x8 a, b;
+  __builtin_memcpy(&a, ha, 16);
+  __builtin_memcpy(&b, hb, 16);
+  f32x16 acc = {};
+  acc = mfma16_bf16(a, b, acc);
+  for (int i = 0; i < 16; ++i) D[(8 * (i >> 2) + 4 * g + (i & 3)) * 32 + m] = acc[i];
+}
+
+// LDS pitches of k_conv_fwd in BF16 elements: odd multiples of 8 (16 bytes),
+// so that the 16-byte reads of 16 consecutive lanes hit distinct banks.
+__host__ __device__ constexpr int odd8(int n) {
+  return ((n + 7) / 8) % 2 ? (n + 7) / 8 * 8 : (n + 7) / 8 * 8 + 8;
+}
+__host__ __device__ constexpr int cnn_kp(int ci) { return odd8(kCnnTaps * ci); }
+__host__ __device__ constexpr int cnn_ps(int ci) { return odd8(ci); }
 
 // LDS bytes of k_conv_fwd: weights [32][KP] + patch [R + 4][W + 4][PS]
 __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
@@ -71,12 +113,16 @@ __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
 
 // One workgroup (4 waves) per (image, R output rows); each wave computes
 // 32 output channels x 32 pixels tiles.
-template <int CI>
+// Epilogue: FUSE = 0 writes the FP32 pre-activations y (co_out channels);
+// FUSE = 1 writes the BF16 GELU activations a[p][32] of a hidden layer
+// instead (co_out = 32); FUSE = 2 writes both.
+template <int CI, int FUSE = 0>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
                                                   const uint16_t* __restrict__ w,
                                                   const float* __restrict__ bias,
                                                   float* __restrict__ y, int H, int W,
-                                                  int co_out, int R) {
+                                                  int co_out, int R,
+                                                  uint16_t* __restrict__ a_out = nullptr) {
   constexpr int K = kCnnTaps * CI;
   constexpr int KP = cnn_kp(CI);
   constexpr int PS = cnn_ps(CI);
@@ -86,20 +132,49 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x
   const int WP = W + 4;
   const int nyb = H / R;
   const int b = blockIdx.x / nyb, y0 = (blockIdx.x % nyb) * R;
-  for (int t = threadIdx.x; t < 32 * K / 4; t += blockDim.x) {
-    const int co = (4 * t) / K, k = (4 * t) % K;
-    *reinterpret_cast<uint2*>(&sw[co * KP + k]) =
-        *reinterpret_cast<const uint2*>(&w[co * K + k]);
-  }
-  constexpr int NV = CI / 4;  // 8-byte vectors per pixel
-  const int nrow = R + 4;
-  for (int t = threadIdx.x; t < nrow * WP * NV; t += blockDim.x) {
-    const int v = t % NV, rest = t / NV;
-    const int c = rest % WP, r = rest / WP;
-    const int gy = (y0 - 2 + r + H) % H;
-    const int gx = (c - 2 + W) % W;
-    *reinterpret_cast<uint2*>(&sp[(r * WP + c) * PS + 4 * v]) =
-        *reinterpret_cast<const uint2*>(&x[(((size_t)b * H + gy) * W + gx) * CI + 4 * v]);
+  const int nt = blockDim.x;
+  // Staging: 16-byte global loads and LDS stores (K, CI and both pitches
+  // are multiples of 8), kLoads weight and kLoads patch loads in flight per
+  // thread before the stores. Weights: 32 rows of K. Patch: rows
+  // y0 - 2 .. y0 + R + 1, columns -2 .. W + 1 (periodic), pixels of PS;
+  // element t = (r WP + c) NV + q. A thread's elements are t0 + j nt: q is
+  // fixed (nt is a multiple of NV) and (r, c) advance by nt / NV columns
+  // with a row carry, so no division runs per element.
+  constexpr int kLoads = 16;
+  constexpr int NW16 = 32 * K / 8;
+  constexpr int NV = CI / 8;  // 16-byte vectors per pixel
+  const int NP16 = (R + 4) * WP * NV;
+  const int q = threadIdx.x % NV, cstep = nt / NV;
+  int pc = (threadIdx.x / NV) % WP, pr = (threadIdx.x / NV) / WP;
+  for (int t0 = threadIdx.x; t0 < NW16 || t0 < NP16; t0 += kLoads * nt) {
+    u32x4 vw[kLoads], vp[kLoads];
+    int off[kLoads];
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+      const int t = t0 + j * nt;
+      if (t < NW16) vw[j] = reinterpret_cast<const u32x4*>(w)[t];
+      off[j] = (pr * WP + pc) * PS + 8 * q;
+      if (t < NP16) {
+        int gy = y0 - 2 + pr, gx = pc - 2;
+        gy += gy < 0 ? H : (gy >= H ? -H : 0);
+        gx += gx < 0 ? W : (gx >= W ? -W : 0);
+        vp[j] = *reinterpret_cast<const u32x4*>(&x[(((size_t)b * H + gy) * W + gx) * CI + 8 * q]);
+      }
+      pc += cstep;
+      while (pc >= WP) {
+        pc -= WP;
+        ++pr;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+      const int t = t0 + j * nt;
+      if (t < NW16) {
+        const int co = (8 * t) / K, k = (8 * t) % K;
+        *reinterpret_cast<u32x4*>(&sw[co * KP + k]) = vw[j];
+      }
+      if (t < NP16) *reinterpret_cast<u32x4*>(&sp[off[j]]) = vp[j];
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -108,21 +183,51 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x
   for (int tile = wid; tile < R * nchunk; tile += nw) {
     const int r = tile / nchunk, n0 = (tile % nchunk) * 32;
     f32x16 acc = {};
-#pragma unroll 4
-    for (int k0 = 0; k0 < K; k0 += 8) {
-      const int k = k0 + 4 * g;
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&sw[m * KP + k]);
+    // k = (tap, ci): 8 consecutive k of a lane share one tap (CI % 8 == 0)
+#pragma unroll 5
+    for (int k0 = 0; k0 + 16 <= K; k0 += 16) {
+      const int k = k0 + 8 * g;
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
       const int tap = k / CI, ci = k % CI;
       const int dy = tap / 5, dx = tap % 5;
-      const bf16x4 bb =
-          *reinterpret_cast<const bf16x4*>(&sp[((r + dy) * WP + n0 + m + dx) * PS + ci]);
-      acc = mfma_bf16(a, bb, acc);
+      const bf16x8 bb =
+          *reinterpret_cast<const bf16x8*>(&sp[((r + dy) * WP + n0 + m + dx) * PS + ci]);
+      acc = mfma16_bf16(a, bb, acc);
     }
-    float* yo = y + (((size_t)b * H + y0 + r) * W + n0 + m) * co_out;
+    if constexpr (K % 16 != 0) {
+      // last 8 k: the upper half-wave supplies zeros
+      const int k = K - 8;
+      bf16x8 a = {}, bb = {};
+      if (g == 0) {
+        const int tap = k / CI, ci = k % CI;
+        a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+        bb = *reinterpret_cast<const bf16x8*>(
+            &sp[((r + tap / 5) * WP + n0 + m + tap % 5) * PS + ci]);
+      }
+      acc = mfma16_bf16(a, bb, acc);
+    }
+    const size_t pix = ((size_t)b * H + y0 + r) * W + n0 + m;
+    if (FUSE != 1) {
+      float* yo = y + pix * co_out;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int co = 8 * (i >> 2) + 4 * g + (i & 3);
-      if (co < co_out) yo[co] = acc[i] + bias[co];
+      for (int i = 0; i < 16; ++i) {
+        const int co = 8 * (i >> 2) + 4 * g + (i & 3);
+        if (co < co_out) yo[co] = acc[i] + bias[co];
+      }
+    }
+    if (FUSE != 0) {
+      // channels 8 j + 4 g .. + 3 of this lane are contiguous: one 8-byte store
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int co = 8 * j + 4 * g;
+        uint16_t h[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) h[q] = to_bf16(gelu(acc[4 * j + q] + bias[co + q]));
+        uint2 v;
+        v.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+        v.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+        *reinterpret_cast<uint2*>(&a_out[pix * 32 + co]) = v;
+      }
     }
   }
 }
@@ -224,13 +329,32 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__
   }
 }
 
-// out[i] = sum_g part[g][i] (fixed order), i < n
-__global__ void k_cnn_reduce(const float* part, int ngroups, int n, float* out) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float s = 0.0f;
-    for (int gq = 0; gq < ngroups; ++gq) s += part[(size_t)gq * n + i];
-    out[i] = s;
+// out[i] = sum_g part[g][i], i < n. A workgroup (4 waves) owns 64
+// consecutive outputs; wave w sums the groups w, w + 4, ... into 8
+// interleaved partial sums (8 loads in flight per lane), which are combined
+// in a fixed order, then the 4 waves in a fixed order: deterministic, the
+// same result for every launch. Launch with (n + 63) / 64 workgroups of 256.
+__global__ __launch_bounds__(256) void k_cnn_reduce(const float* part, int ngroups, int n,
+                                                    float* out) {
+  __shared__ float red[4][64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  float acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
+  if (i < n) {
+    int gq = wid;
+    for (; gq + 28 < ngroups; gq += 32) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] += part[(size_t)(gq + 4 * j) * n + i];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (gq + 4 * j < ngroups) acc[j] += part[(size_t)(gq + 4 * j) * n + i];
   }
+  red[wid][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  __syncthreads();
+  if (wid == 0 && i < n) out[i] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 // bias gradient partials: part[blk][o] = sum over the block's pixels of dy[p][o]
@@ -251,14 +375,6 @@ __global__ __launch_bounds__(256) void k_cnn_bias_grad(const uint16_t* dy, long 
   }
 }
 
-__device__ __forceinline__ float gelu(float v) {
-  return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
-}
-__device__ __forceinline__ float gelu_grad(float v) {
-  const float cdf = 0.5f * (1.0f + erff(v * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * v * v);
-  return cdf + v * pdf;
-}
 
 // a[p][c] = BF16(act(y[p][c])) for c < cin, 0 up to cpad; act = GELU
 // (gelu != 0) or identity

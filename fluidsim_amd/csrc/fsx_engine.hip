@@ -56,7 +56,8 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // stream it runs on, with the algorithmic bytes of that launch (compulsory
 // HBM reads + writes of its inputs and outputs).
 struct fsx_timing {
-  static constexpr int kKinds = 8;
+  static constexpr int kKinds = 9;     // kind 8: hidden-layer convolutions
+  static constexpr int kReadKinds = 8;  // kinds returned by fsx_timing_read
   bool on = false;
   std::vector<hipEvent_t> ev;  // pairs
   std::vector<int> kind;
@@ -114,7 +115,7 @@ namespace {
 
 // kinds of launches reported by fsx_timing_read
 enum { K_ZINV = 0, K_YINV = 1, K_XPASS = 2, K_YFWD = 3, K_ZFWD = 4, K_RK = 5,
-       K_OTHER = 6, K_DIAG = 7 };
+       K_OTHER = 6, K_DIAG = 7, K_CNN = 8 };
 
 thread_local fsx_plan_s* g_plan = nullptr;  // plan of the running API call
 thread_local int g_kind = K_OTHER;
@@ -1785,19 +1786,39 @@ int fsx_timing_read(fsx_plan p, double* ms, double* bytes, long long* counts,
                     int reset) {
   if (!p || !ms || !bytes || !counts) return fail(FSX_ERR_ARG, "bad argument");
   if (!p->tm) {
-    for (int k = 0; k < fsx_timing::kKinds; ++k) ms[k] = bytes[k] = 0, counts[k] = 0;
+    for (int k = 0; k < fsx_timing::kReadKinds; ++k) ms[k] = bytes[k] = 0, counts[k] = 0;
     return 0;
   }
   int rc = tm_flush(p);
   if (rc) return rc;
   for (int k = 0; k < fsx_timing::kKinds; ++k) {
-    ms[k] = p->tm->ms[k];
-    bytes[k] = p->tm->by[k];
-    counts[k] = p->tm->cnt[k];
+    if (k < fsx_timing::kReadKinds) {
+      ms[k] = p->tm->ms[k];
+      bytes[k] = p->tm->by[k];
+      counts[k] = p->tm->cnt[k];
+    }
     if (reset) {
       p->tm->ms[k] = p->tm->by[k] = 0;
       p->tm->cnt[k] = 0;
     }
+  }
+  return 0;
+}
+
+int fsx_timing_read_network(fsx_plan p, double* ms, double* flops, long long* count,
+                            int reset) {
+  if (!p || !ms || !flops || !count) return fail(FSX_ERR_ARG, "bad argument");
+  *ms = *flops = 0;
+  *count = 0;
+  if (!p->tm) return 0;
+  int rc = tm_flush(p);
+  if (rc) return rc;
+  *ms = p->tm->ms[K_CNN];
+  *flops = p->tm->by[K_CNN];
+  *count = p->tm->cnt[K_CNN];
+  if (reset) {
+    p->tm->ms[K_CNN] = p->tm->by[K_CNN] = 0;
+    p->tm->cnt[K_CNN] = 0;
   }
   return 0;
 }
@@ -2155,11 +2176,23 @@ int cnn_grid(long long n) {
   return (int)std::max<long long>(1, std::min<long long>(b, 4096));
 }
 
+// fuse: 0 -> FP32 pre-activations y; 1 -> BF16 GELU activations a (32
+// channels, co_out = 32); 2 -> both
+template <int CI, int FUSE>
+void conv_fwd_launch(unsigned nblk, size_t lds, hipStream_t s, const uint16_t* x,
+                     const uint16_t* w, const float* bias, float* y, int H, int W, int co_out,
+                     int R, uint16_t* a) {
+  hipLaunchKernelGGL((k_conv_fwd<CI, FUSE>), dim3(nblk), dim3(256), lds, s, x, w, bias, y, H, W,
+                     co_out, R, a);
+}
+
 int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, const float* bias,
-                 int co_out, float* y, hipStream_t s) {
-  if (!x || !w || !bias || !y || B < 1) return fail(FSX_ERR_ARG, "bad argument");
+                 int co_out, float* y, hipStream_t s, int fuse = 0, void* a = nullptr) {
+  if (!x || !w || !bias || B < 1 || fuse < 0 || fuse > 2) return fail(FSX_ERR_ARG, "bad argument");
+  if ((fuse != 1 && !y) || (fuse != 0 && !a)) return fail(FSX_ERR_ARG, "bad argument");
   if (!cnn_channels_ok(ci)) return fail(FSX_ERR_ARG, "input channels must be 8 or 32");
   if (co_out < 1 || co_out > 32) return fail(FSX_ERR_ARG, "output channels must be 1..32");
+  if (fuse && co_out != 32) return fail(FSX_ERR_ARG, "fused activations need 32 output channels");
   if (H < kCnnRows || H % kCnnRows || W < 32 || W % 32)
     return fail(FSX_ERR_ARG, "H must be a multiple of 4 and W a multiple of 32");
   // rows per workgroup: 4 when there are enough workgroups, fewer for
@@ -2171,18 +2204,23 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   const unsigned nblk = (unsigned)(B * (H / R));
   const auto* xi = reinterpret_cast<const uint16_t*>(x);
   const auto* wi = reinterpret_cast<const uint16_t*>(w);
+  auto* ai = reinterpret_cast<uint16_t*>(a);
   int rc;
-  if (ci == 8) {
-    if ((rc = set_lds(k_conv_fwd<8>, conv_fwd_lds(8, W, kCnnRows)))) return rc;
-    hipLaunchKernelGGL(k_conv_fwd<8>, dim3(nblk), dim3(256), lds, s, xi, wi, bias, y, H, W,
-                       co_out, R);
-  } else {
-    if ((rc = set_lds(k_conv_fwd<32>, conv_fwd_lds(32, W, kCnnRows)))) return rc;
-    hipLaunchKernelGGL(k_conv_fwd<32>, dim3(nblk), dim3(256), lds, s, xi, wi, bias, y, H, W,
-                       co_out, R);
+#define FSX_CONV_FWD(CI_, F_)                                                           \
+  if (ci == CI_ && fuse == F_) {                                                        \
+    if ((rc = set_lds(k_conv_fwd<CI_, F_>, conv_fwd_lds(CI_, W, kCnnRows)))) return rc; \
+    if ((rc = tm_pre(s))) return rc;                                                    \
+    conv_fwd_launch<CI_, F_>(nblk, lds, s, xi, wi, bias, y, H, W, co_out, R, ai);       \
   }
+  FSX_CONV_FWD(8, 0)
+  FSX_CONV_FWD(8, 1)
+  FSX_CONV_FWD(8, 2)
+  FSX_CONV_FWD(32, 0)
+  FSX_CONV_FWD(32, 1)
+  FSX_CONV_FWD(32, 2)
+#undef FSX_CONV_FWD
   FSX_HIP(hipGetLastError());
-  return 0;
+  return tm_post(s);
 }
 
 int cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,
@@ -2192,6 +2230,45 @@ int cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, voi
                      cpad, use_gelu, reinterpret_cast<uint16_t*>(a));
   FSX_HIP(hipGetLastError());
   return 0;
+}
+
+// two BF16 activation buffers of 32 channels
+long long cnn_forward_work(int B, int H, int W) { return 2LL * B * H * W * 32 * 2; }
+
+// Network forward: hidden layers write their BF16 GELU activations from the
+// convolution epilogue (ping-pong between the two buffers of work), the last
+// layer writes its FP32 outputs to y.
+int cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
+                const float* const* b, const void* x, int B, int H, int W, float* y,
+                void* work, hipStream_t s) {
+  const long long npix = (long long)B * H * W;
+  uint16_t* act[2] = {reinterpret_cast<uint16_t*>(work),
+                      reinterpret_cast<uint16_t*>(work) + npix * 32};
+  const void* in = x;
+  for (int l = 0; l < nl; ++l) {
+    const bool last = l == nl - 1;
+    if (!cnn_channels_ok(cin[l])) return fail(FSX_ERR_ARG, "layer input channels must be 8 or 32");
+    if (!w[l] || !b[l]) return fail(FSX_ERR_ARG, "bad argument");
+    if (!last && cout[l] != 32) return fail(FSX_ERR_ARG, "hidden layers have 32 channels");
+    // timing: hidden 32 -> 32 layers as their own kind, work in flops
+    if (l > 0 && !last)
+      note(K_CNN, 2.0 * npix * 32 * kCnnTaps * 32);
+    else
+      note(K_OTHER, 0.0);
+    const int rc = last ? cnn_conv_fwd(in, B, H, W, cin[l], w[l], b[l], cout[l], y, s)
+                        : cnn_conv_fwd(in, B, H, W, cin[l], w[l], b[l], 32, nullptr, s, 1,
+                                       act[l & 1]);
+    if (rc) return rc;
+    in = act[l & 1];
+  }
+  return 0;
+}
+
+// workspace of fsx_ns2d_closure_term: vorticity and network output (FP32),
+// the BF16 network input (8 channels), the network's activations
+long long closure_work(const fsx_plan_s* p) {
+  const long long npix = (long long)p->ny * p->nx;
+  return npix * (4 + 4 + 16) + cnn_forward_work(1, p->ny, p->nx);
 }
 
 }  // namespace
@@ -2207,9 +2284,23 @@ int fsx_cnn_probe(const void* A, const void* B, float* D, void* stream) {
   return 0;
 }
 
+int fsx_cnn_probe16(const void* A, const void* B, float* D, void* stream) {
+  if (!A || !B || !D) return fail(FSX_ERR_ARG, "bad argument");
+  hipLaunchKernelGGL(k_mfma16_probe, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                     reinterpret_cast<const uint16_t*>(A), reinterpret_cast<const uint16_t*>(B),
+                     D);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
 int fsx_cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w,
                      const float* bias, int co_out, float* y, void* stream) {
   return cnn_conv_fwd(x, B, H, W, ci, w, bias, co_out, y, (hipStream_t)stream);
+}
+
+int fsx_cnn_conv_fwd_act(const void* x, int B, int H, int W, int ci, const void* w,
+                         const float* bias, float* y, void* a, void* stream) {
+  return cnn_conv_fwd(x, B, H, W, ci, w, bias, 32, y, (hipStream_t)stream, y ? 2 : 1, a);
 }
 
 int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void* stream) {
@@ -2267,7 +2358,7 @@ int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, 
 #undef FSX_WG
   FSX_HIP(hipGetLastError());
   const int n = 32 * kCnnTaps * ci;
-  hipLaunchKernelGGL(k_cnn_reduce, dim3(cnn_grid(n)), dim3(256), 0, s, part, groups, n, dw);
+  hipLaunchKernelGGL(k_cnn_reduce, dim3((n + 63) / 64), dim3(256), 0, s, part, groups, n, dw);
   FSX_HIP(hipGetLastError());
   float* bpart = part + (size_t)groups * n;
   const long long npix = (long long)B * H * W;
@@ -2329,33 +2420,47 @@ int fsx_cnn_to_bf16(const float* in, int n, void* out, void* stream) {
   return 0;
 }
 
+int fsx_ns2d_closure_workspace(fsx_plan p, long long* bytes) {
+  if (!p || !bytes) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 2) return fail(FSX_ERR_ARG, "the closure term needs a 2D plan");
+  *bytes = closure_work(p);
+  return 0;
+}
+
+int fsx_ns2d_closure_term(fsx_plan p, int nl, const int* cin, const int* cout,
+                          const void* const* w, const float* const* b, const void* rot_fft,
+                          void* out_fft, void* work, long long work_bytes, void* stream) {
+  PlanScope scope_(p);
+  if (!p || nl < 1 || nl > 16 || !cin || !cout || !w || !b || !rot_fft || !out_fft || !work)
+    return fail(FSX_ERR_ARG, "bad argument");
+  if (p->dim != 2) return fail(FSX_ERR_ARG, "the closure term needs a 2D plan");
+  if (p->prec != 4) return fail(FSX_ERR_ARG, "the learned closure runs on float32 plans");
+  if (cin[0] != 8 || cout[nl - 1] != 1)
+    return fail(FSX_ERR_ARG, "the network maps 1 input channel (padded to 8) to 1 output");
+  if (work_bytes < closure_work(p)) return fail(FSX_ERR_ARG, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  const long long npix = (long long)p->ny * p->nx;
+  char* wb = reinterpret_cast<char*>(work);
+  float* X = reinterpret_cast<float*>(wb);
+  float* Y = reinterpret_cast<float*>(wb + npix * 4);
+  void* X8 = wb + npix * 8;
+  void* fw = wb + npix * 24;
+  int rc = do_ifft<float>(p, rot_fft, X, 1, s);
+  if (!rc) rc = cnn_act(X, npix, 1, 8, 0, X8, s);
+  if (!rc) rc = cnn_forward(nl, cin, cout, w, b, X8, 1, p->ny, p->nx, Y, fw, s);
+  if (!rc) rc = do_fft<float>(p, Y, out_fft, 1, s);
+  if (!rc) rc = fsx_dealias(p, out_fft, 1, 0, stream);
+  return rc;
+}
+
 int fsx_cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
                     const float* const* b, const void* x, int B, int H, int W, float* y,
                     void* work, long long work_bytes, void* stream) {
   if (nl < 1 || nl > 16 || !cin || !cout || !w || !b || !x || !y || !work)
     return fail(FSX_ERR_ARG, "bad argument");
-  const long long npix = (long long)B * H * W;
-  // two BF16 activation buffers of 32 channels and one FP32 pre-activation
-  const long long need = 2 * npix * 32 * 2 + npix * 32 * 4;
-  if (work_bytes < need) return fail(FSX_ERR_ARG, "workspace too small");
   hipStream_t s = (hipStream_t)stream;
-  uint16_t* act[2] = {reinterpret_cast<uint16_t*>(work),
-                      reinterpret_cast<uint16_t*>(work) + npix * 32};
-  float* pre = reinterpret_cast<float*>(reinterpret_cast<char*>(work) + 2 * npix * 32 * 2);
-  const void* in = x;
-  for (int l = 0; l < nl; ++l) {
-    const bool last = l == nl - 1;
-    if (!cnn_channels_ok(cin[l])) return fail(FSX_ERR_ARG, "layer input channels must be 8 or 32");
-    if (!last && cout[l] != 32) return fail(FSX_ERR_ARG, "hidden layers have 32 channels");
-    int rc = cnn_conv_fwd(in, B, H, W, cin[l], w[l], b[l], cout[l], last ? y : pre, s);
-    if (rc) return rc;
-    if (!last) {
-      rc = cnn_act(pre, npix, 32, 32, 1, act[l & 1], s);
-      if (rc) return rc;
-      in = act[l & 1];
-    }
-  }
-  return 0;
+  if (work_bytes < cnn_forward_work(B, H, W)) return fail(FSX_ERR_ARG, "workspace too small");
+  return cnn_forward(nl, cin, cout, w, b, x, B, H, W, y, work, s);
 }
 
 }  // extern "C"

@@ -367,10 +367,16 @@ int fsx_graph_destroy(fsx_graph g);
  * Channels-last layouts: activations BF16 [B][H][W][C] with C = 8 or 32
  * (zero-padded), pre-activations FP32 [B][H][W][co], weights BF16
  * [32][25][C] (tap t = 5 dy + dx, offset (dy - 2, dx - 2), periodic),
- * bias FP32 [32]; MFMA 32x32x8 BF16 with FP32 accumulation.
+ * bias FP32 [32]; MFMA 32x32x16 (convolutions) and 32x32x8 (weight
+ * gradients) BF16 with FP32 accumulation.
  * fsx_cnn_probe: D (32x32 FP32) = A (32x8 BF16) B (8x32 BF16), one wave.
+ * fsx_cnn_probe16: the same with A (32x16) and B (16x32) through the
+ *   K = 16 instruction the convolutions use.
  * fsx_cnn_conv_fwd: y = bias + periodic 5x5 correlation of x, co_out <= 32
  *   outputs; H multiple of 4, W multiple of 32, W <= 128 for C = 32.
+ * fsx_cnn_conv_fwd_act: a hidden layer (32 outputs) with the activation in
+ *   the epilogue: a = BF16(GELU(y)) (32 channels) and, when y is not NULL,
+ *   the FP32 pre-activations y the backward pass needs.
  * fsx_cnn_conv_flip: wt[c][24 - t][o] = w[o][t][c] (co_pad = 8 or 32
  *   input channels of wt), so fsx_cnn_conv_fwd(dy, wt) is the input gradient.
  * fsx_cnn_conv_wgrad: dw[o][t][c] = sum_p dy[p][o] x[p + d(t)][c] and
@@ -384,10 +390,20 @@ int fsx_graph_destroy(fsx_graph g);
  *   optional BF16 copy into wb.
  * fsx_cnn_to_bf16: BF16 copy of n FP32 values.
  * fsx_cnn_forward: nl layers (GELU between them) in one call, inference;
- *   work holds 2 x B H W x 32 BF16 + B H W x 32 FP32. */
+ *   hidden layers write their BF16 activations from the convolution
+ *   epilogue; work holds 2 x B H W x 32 BF16.
+ * fsx_ns2d_closure_term: the learned subgrid term of a float32 ns2d plan in
+ *   one call: c2r of rot_fft, the network (cin[0] = 8, cout[nl-1] = 1) on
+ *   the vorticity, r2c into out_fft and dealiasing; the term enters the
+ *   Runge-Kutta stages through fsx_ns2d_rk_stage_ext, as fluidsim adds a
+ *   model term to the tendencies (fluidsim/solvers/ns3d/solver.py:246-249);
+ *   workspace size from fsx_ns2d_closure_workspace. */
 int fsx_cnn_probe(const void* A, const void* B, float* D, void* stream);
+int fsx_cnn_probe16(const void* A, const void* B, float* D, void* stream);
 int fsx_cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w,
                      const float* bias, int co_out, float* y, void* stream);
+int fsx_cnn_conv_fwd_act(const void* x, int B, int H, int W, int ci, const void* w,
+                         const float* bias, float* y, void* a, void* stream);
 int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void* stream);
 int fsx_cnn_wgrad_workspace(int B, int H, int ci, long long* bytes);
 int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W,
@@ -405,6 +421,10 @@ int fsx_cnn_to_bf16(const float* in, int n, void* out, void* stream);
 int fsx_cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
                     const float* const* b, const void* x, int B, int H, int W, float* y,
                     void* work, long long work_bytes, void* stream);
+int fsx_ns2d_closure_workspace(fsx_plan p, long long* bytes);
+int fsx_ns2d_closure_term(fsx_plan p, int nl, const int* cin, const int* cout,
+                          const void* const* w, const float* const* b, const void* rot_fft,
+                          void* out_fft, void* work, long long work_bytes, void* stream);
 
 /* Live per-kernel timing (no reference counterpart: measurement support).
  * When enabled, HIP events are recorded around every pass launch on its
@@ -417,6 +437,12 @@ int fsx_cnn_forward(int nl, const int* cin, const int* cout, const void* const* 
 int fsx_timing_enable(fsx_plan p, int enable);
 int fsx_timing_read(fsx_plan p, double* ms8, double* bytes8, long long* counts8,
                     int reset);
+/* fsx_timing_read_network (sync): the same for the hidden 32 -> 32
+ * convolutions of fsx_ns2d_closure_term issued with this plan, work in
+ * flops (2 x pixels x 32 x 25 x 32 per launch); reset zeroes them (the
+ * reset of fsx_timing_read does too). */
+int fsx_timing_read_network(fsx_plan p, double* ms, double* flops, long long* count,
+                            int reset);
 
 #ifdef __cplusplus
 }

@@ -140,3 +140,50 @@ def test_small_pipeline_runs():
     CL.run_steps(les, 20)
     E = CL.energy_spectrum(les.oper, les.state.state_spect.data)
     assert np.all(np.isfinite(E)) and E.sum() > 0
+
+
+def test_closure_term_one_call():
+    """fsx_ns2d_closure_term equals the composition of the separately tested
+    pieces (c2r, network forward, r2c, dealiasing) bit for bit, and the FP64
+    oracle (ifft, network with BF16-rounded activations, fft, dealiasing)
+    within 5e-3 relative L2; a float64 plan is refused."""
+    from .test_gpu_cnn import forward_bf16_acts, oracle_net
+
+    n = 64
+    SimulLES = CL.make_les_class()
+    _, params = CL.ns2d_params(n, 1e-3, forcing=False, les=True)
+    sim = SimulLES(params)
+    sim.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(n, n, seed=8, kmax=16))
+    net = CL.CNNClosure(n_in=1, nlayers=4, seed=3)
+    rng = np.random.default_rng(10)
+    for t in net.b:
+        t.copy_(torch.from_numpy(rng.standard_normal(32).astype(np.float32) * 0.1))
+    sim.closure_net = net
+    S = sim.state.state_spect.data
+    got = sim._closure_term(S).clone()
+
+    oper = sim.oper
+    x = torch.empty((1, n, n, 1), dtype=torch.float32, device="cuda")
+    oper.ifft_as_arg(S[0], x.view(n, n))
+    y = net.forward(x)
+    ref = torch.empty_like(got)
+    oper.fft_as_arg(y.view(n, n), ref[0])
+    oper.dealiasing(ref)
+    assert torch.equal(got, ref)
+
+    o = Oper2D(n, n)
+    w = o.ifft(S[0].cpu().numpy().astype(np.complex128))
+    xin = np.zeros((1, n, n, 8))
+    xin[0, :, :, 0] = w
+    out = forward_bf16_acts(oracle_net(net), xin)[0]
+    K = o.fft(out)
+    o.dealiasing(K)
+    err = rel_l2(got[0].cpu().numpy(), K)
+    print(f"closure term vs oracle rel L2 = {err:.2e}")
+    assert err < 5e-3
+
+    _, p64 = CL.ns2d_params(n, 1e-3, precision="float64", forcing=False, les=True)
+    sim64 = SimulLES(p64)
+    sim64.closure_net = net
+    with pytest.raises(ValueError):
+        sim64._closure_term(sim64.state.state_spect.data)

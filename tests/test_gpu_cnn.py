@@ -45,6 +45,17 @@ def test_mfma_layout_probe():
     assert np.array_equal(D.cpu().numpy(), A @ B)
 
 
+def test_mfma16_layout_probe():
+    """Operand layout of the K = 16 instruction the convolutions use."""
+    rng = np.random.default_rng(11)
+    A = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    B = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    D = torch.zeros((32, 32), dtype=torch.float32, device="cuda")
+    a, b = dev(bf16_bits(A)), dev(bf16_bits(B))
+    _lib.call("fsx_cnn_probe16", _lib.dptr(a), _lib.dptr(b), _lib.dptr(D), _lib.stream_ptr())
+    assert np.array_equal(D.cpu().numpy(), A @ B)
+
+
 def conv_gpu(x, w, b, co_out):
     B_, H, W, ci = x.shape
     y = torch.empty((B_, H, W, co_out), dtype=torch.float32, device="cuda")
@@ -172,3 +183,51 @@ def test_training_step_gradients_and_descent():
     losses = [net.train_step(dev(x), dev(target.astype(np.float32))) for _ in range(200)]
     print(f"loss {loss:.3e} -> {losses[-1]:.3e}")
     assert losses[-1] < 0.1 * loss
+
+
+@pytest.mark.parametrize("ci", [8, 32])
+def test_fused_activation_epilogue_is_bit_exact(ci):
+    """fsx_cnn_conv_fwd_act (GELU in the convolution epilogue) writes the
+    same pre-activations and BF16 activations as fsx_cnn_conv_fwd followed
+    by fsx_cnn_act."""
+    rng = np.random.default_rng(8)
+    B_, H, W = 2, 8, 64
+    x = dev(bf16_bits(rng.standard_normal((B_, H, W, ci))))
+    w = dev(bf16_bits(rng.standard_normal((32, 25, ci)) * 0.2))
+    b = dev(rng.standard_normal(32).astype(np.float32))
+    s = _lib.stream_ptr()
+    y0 = torch.empty((B_, H, W, 32), dtype=torch.float32, device="cuda")
+    a0 = torch.empty((B_, H, W, 32), dtype=torch.int16, device="cuda")
+    _lib.call("fsx_cnn_conv_fwd", _lib.dptr(x), B_, H, W, ci, _lib.dptr(w), _lib.dptr(b), 32,
+              _lib.dptr(y0), s)
+    _lib.call("fsx_cnn_act", _lib.dptr(y0), B_ * H * W, 32, 32, 1, _lib.dptr(a0), s)
+    y1 = torch.full_like(y0, np.nan)
+    a1 = torch.zeros_like(a0)
+    a2 = torch.zeros_like(a0)
+    _lib.call("fsx_cnn_conv_fwd_act", _lib.dptr(x), B_, H, W, ci, _lib.dptr(w), _lib.dptr(b),
+              _lib.dptr(y1), _lib.dptr(a1), s)
+    _lib.call("fsx_cnn_conv_fwd_act", _lib.dptr(x), B_, H, W, ci, _lib.dptr(w), _lib.dptr(b),
+              None, _lib.dptr(a2), s)
+    assert torch.equal(y0, y1)
+    assert torch.equal(a0, a1) and torch.equal(a0, a2)
+
+
+def test_network_forward_matches_layer_by_layer():
+    """fsx_cnn_forward (fused hidden layers) equals the training path's
+    layer-by-layer forward bit for bit."""
+    rng = np.random.default_rng(9)
+    net = CNNClosure(n_in=1, nlayers=5, seed=2)
+    for t in net.b:
+        t.copy_(torch.from_numpy(rng.standard_normal(32).astype(np.float32) * 0.1))
+    x = dev(rng.standard_normal((2, 16, 64, 1)).astype(np.float32))
+    got = net.forward(x)
+    s = _lib.stream_ptr()
+    a = net.pack_input(x).clone()
+    for l in range(net.nlayers):
+        y = torch.empty((2, 16, 64, net.cout[l]), dtype=torch.float32, device="cuda")
+        _lib.call("fsx_cnn_conv_fwd", _lib.dptr(a), 2, 16, 64, net.cin[l], _lib.dptr(net.wb[l]),
+                  _lib.dptr(net.b[l]), net.cout[l], _lib.dptr(y), s)
+        if l < net.nlayers - 1:
+            a = torch.empty((2, 16, 64, 32), dtype=torch.int16, device="cuda")
+            _lib.call("fsx_cnn_act", _lib.dptr(y), 2 * 16 * 64, 32, 32, 1, _lib.dptr(a), s)
+    assert torch.equal(got, y[..., 0])

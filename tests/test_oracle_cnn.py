@@ -84,3 +84,33 @@ def test_planted_mistake_unflipped_kernel(monkeypatch):
     d = rng.standard_normal(x.shape)
     fd = (net.loss(x + eps * d, target) - net.loss(x - eps * d, target)) / (2 * eps)
     assert abs(np.sum(dx * d) - fd) > 1e-3 * abs(fd)
+
+
+def test_training_symmetries_of_the_ns2d_tendency():
+    """The closure training turns and mirrors its samples: the ns2d nonlinear
+    tendency must commute with a 90-degree turn of a square periodic box and
+    with a mirror that flips the sign of the vorticity (pseudoscalar)."""
+    from oracle.operators import Oper2D
+    from oracle.solvers import NS2D
+
+    n = 32
+    oper = Oper2D(n, n)
+    sol = NS2D(oper)
+    rng = np.random.default_rng(3)
+    w_fft = oper.fft(rng.standard_normal(oper.shapeX))
+    oper.dealiasing(w_fft)
+    w = oper.ifft(w_fft)
+
+    def tend(wp):
+        return oper.ifft(sol.tendencies_nonlin(np.array([oper.fft(wp)]))[0])
+
+    N = tend(w)
+    scale = np.abs(N).max()
+    for turn in (1, 2, 3):
+        got = tend(np.rot90(w, turn).copy())
+        assert np.abs(got - np.rot90(N, turn)).max() < 1e-12 * scale
+    got = tend(-np.flip(w, 1).copy())
+    assert np.abs(got + np.flip(N, 1)).max() < 1e-12 * scale
+    # a mirror without the sign change is not a symmetry
+    bad = tend(np.flip(w, 1).copy())
+    assert np.abs(bad - np.flip(N, 1)).max() > 1e-3 * scale
