@@ -30,6 +30,7 @@ the algorithmic bytes B_alg = T (b_X + b_K) + R b_K per RK4 step
 
 import argparse
 import contextlib
+import gc
 import json
 import math
 import os
@@ -54,7 +55,7 @@ def b_alg_ns3d(n, prec):
     return 36 * (bX + bK) + 51 * bK
 
 
-def make_sim(n, precision, seed=0):
+def make_sim(n, precision, seed=0, vv=None):
     from fluidsim_amd.solvers.ns3d.solver import Simul
     from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
 
@@ -70,13 +71,297 @@ def make_sim(n, precision, seed=0):
     params.output.periods_print.print_stdout = 0
     sim = Simul(params)
     cdt = np.complex64 if precision == "float32" else np.complex128
-    vv = ns3d_spectrum_vv_fft(n, seed=seed, dtype=cdt)
-    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
-    del vv
+    if vv is None:
+        vv = ns3d_spectrum_vv_fft(n, seed=seed)
+    # the generator rounds its FP64 field at the end: a cast of the FP64
+    # field is the FP32 field of the same seed
+    sim.state.init_statespect_from(vx_fft=vv[0].astype(cdt), vy_fft=vv[1].astype(cdt),
+                                   vz_fft=vv[2].astype(cdt))
     # fixed dt from the initial CFL condition
     sim.time_stepping.compute_time_increment_CLF()
     params.time_stepping.USE_CFL = False
     return sim
+
+
+def b_alg_strat(n, prec):
+    """T (b_X + b_K) + R b_K per RK4 step of ns3d.strat, T = 52, R = 68."""
+    bX = prec * n**3
+    bK = 2 * prec * n * n * (n // 2 + 1)
+    return 52 * (bX + bK) + 68 * bK
+
+
+# anisotropic forcing region of the strat workload (45 degrees, 10 degrees wide)
+STRAT_ANISO = {"angle": np.pi / 4, "delta_angle": np.radians(10)}
+
+
+def make_strat_sim(n, precision, seed=0, fixed_dt=None):
+    """ns3d.strat, N = 4, Pr = 1 (one nu_8 for v and b), nu_8 with
+    k_max eta = 1, toroidal time-correlated random forcing at k in [2, 4]
+    (tcrandom_anisotropic, key_forced = vt_fft), isotropic velocity
+    E(k) ~ k^4 exp(-2 (k/4)^2) (seed), b = 0; CFL 0.5, or a fixed dt."""
+    from fluidsim_amd.solvers.ns3d.strat.solver import Simul
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+
+    params = Simul.create_default_params()
+    po = params.oper
+    po.nx = po.ny = po.nz = n
+    po.Lx = po.Ly = po.Lz = 2 * np.pi
+    po.precision = precision
+    params.N = 4.0
+    params.nu_8 = nu8_for_kmax_eta(n)
+    pt = params.time_stepping
+    pt.USE_CFL = fixed_dt is None
+    pt.cfl_coef = 0.5
+    if fixed_dt is not None:
+        pt.deltat0 = fixed_dt
+    pt.USE_T_END = False
+    params.forcing.enable = True
+    params.forcing.type = "tcrandom_anisotropic"
+    params.forcing.key_forced = "vt_fft"
+    params.forcing.nkmin_forcing = 2
+    params.forcing.nkmax_forcing = 4
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    np.random.seed(seed)  # forcing seeds and renewals
+    sim = Simul(params)
+    cdt = np.complex64 if precision == "float32" else np.complex128
+    vv = ns3d_spectrum_vv_fft(n, seed=seed, dtype=cdt)
+    sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+    return sim
+
+
+def _energy_shells(oper, vv):
+    from oracle.diagnostics import shell_spectrum
+
+    return shell_spectrum(sum(0.5 * np.abs(a) ** 2 for a in vv), oper)
+
+
+def _spectrum_error(E, E_ref, kmax):
+    """relative L2 and max relative error of an energy spectrum over shells
+    1 .. kmax (the dealiased range)"""
+    a, r = E[1: kmax + 1], E_ref[1: kmax + 1]
+    return {"rel_l2": float(np.linalg.norm(a - r) / np.linalg.norm(r)),
+            "max_rel": float(np.max(np.abs(a / r - 1)))}
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _device_rel_l2(torch, a, ref):
+    """relative L2 of two device states (FP64 accumulation, field by field)"""
+    num = den = 0.0
+    for f in range(ref.shape[0]):
+        r = ref[f]
+        num += float(torch.linalg.vector_norm(a[f].to(r.dtype) - r).item() ** 2)
+        den += float(torch.linalg.vector_norm(r).item() ** 2)
+    return (num / den) ** 0.5
+
+
+def ns3d_accuracy(torch, n_bench, steps_bench, vv=None, n_oracle=128, steps_oracle=5,
+                  seed=0):
+    """The FP32 engine against FP64: (1) at n_oracle, FP64 and FP32 engine
+    states after steps_oracle fixed-dt steps against the FP64 oracle on the
+    same seeded fields; (2) at the bench size, the FP32 engine against the
+    FP64 engine after steps_bench steps at the same dt, compared on the
+    device (state, and the shell energy spectrum of the engine's FP64
+    diagnostics pass); the FP64 engine holds the oracle to 1e-11 at every
+    size the tests run."""
+    from oracle import operators as O
+    from oracle.solvers import NS3D
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+
+    res = {}
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    O.set_workers(cores)
+    op = O.Oper3D(n_oracle, n_oracle, n_oracle)
+    v0 = ns3d_spectrum_vv_fft(n_oracle, seed=seed)
+    s64 = make_sim(n_oracle, "float64", seed, vv=v0)
+    dt = s64.time_stepping.deltat
+    orc = NS3D(op, nu_8=nu8_for_kmax_eta(n_oracle), deltat0=dt)
+    orc.state_spect[:] = np.array(v0)
+    orc.run(steps_oracle)
+    ref = orc.state_spect
+    E_ref = _energy_shells(op, ref)
+    for prec in ("float64", "float32"):
+        sim = s64 if prec == "float64" else make_sim(n_oracle, prec, seed, vv=v0)
+        sim.time_stepping.deltat = dt
+        for _ in range(steps_oracle):
+            sim.time_stepping.one_time_step_computation()
+        got = sim.state.state_spect.to_numpy().astype(np.complex128)
+        res[f"{prec}_vs_oracle"] = {
+            "state_rel_l2": _rel_l2(got, ref),
+            "energy_spectrum": _spectrum_error(_energy_shells(op, got), E_ref,
+                                               n_oracle // 3)}
+        del sim
+    res["oracle_case"] = (f"ns3d {n_oracle}^3, {steps_oracle} RK4 steps at the FP64 "
+                          f"initial-CFL dt, same seeded fields, FP64 numpy oracle")
+    del s64
+    # bench size: FP32 against the FP64 engine, on the device
+    if vv is None:
+        vv = ns3d_spectrum_vv_fft(n_bench, seed=seed)
+    s64 = make_sim(n_bench, "float64", seed, vv=vv)
+    s32 = make_sim(n_bench, "float32", seed, vv=vv)
+    s32.time_stepping.deltat = s64.time_stepping.deltat
+    for sim in (s64, s32):
+        for _ in range(steps_bench):
+            sim.time_stepping.one_time_step_computation()
+    torch.cuda.synchronize()
+    E = [sim.output.spectra.compute(kzkh=False)[1]["spectra_E"] for sim in (s32, s64)]
+    res["float32_vs_float64_engine"] = {
+        "case": f"ns3d {n_bench}^3, {steps_bench} RK4 steps at the same dt",
+        "state_rel_l2": _device_rel_l2(torch, s32.state.state_spect.data,
+                                       s64.state.state_spect.data),
+        "energy_spectrum": _spectrum_error(E[0], E[1], n_bench // 3)}
+    del s32, s64
+    gc.collect()
+    return res
+
+
+def strat_oracle_check(n=256, steps=3, seed=0):
+    """FP64 and FP32 engine states of the strat workload at n^3 after
+    `steps` forced steps at the FP64 initial-CFL dt against the FP64 oracle
+    (same forcing draws from numpy's global generator); the oracle's own
+    time per step is the CPU baseline."""
+    from oracle import operators as O
+    from oracle.forcing import TCRandomForcing
+    from oracle.solvers import NS3DStrat
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    O.set_workers(cores)
+    s64 = make_strat_sim(n, "float64", seed)
+    s64.time_stepping.compute_time_increment_CLF()
+    dt = s64.time_stepping.deltat
+    del s64
+    got = {}
+    for prec in ("float64", "float32"):
+        sim = make_strat_sim(n, prec, seed, fixed_dt=dt)
+        sim.time_stepping.prepare_main_loop()
+        for _ in range(steps):
+            sim.time_stepping.one_time_step()
+        got[prec] = sim.state.state_spect.to_numpy().astype(np.complex128)
+        del sim
+    np.random.seed(seed)
+    op = O.Oper3D(n, n, n)
+    orc = NS3DStrat(op, N=4.0, nu_8=nu8_for_kmax_eta(n), deltat0=dt)
+    vv = ns3d_spectrum_vv_fft(n, seed=seed)
+    orc.state_spect[:] = np.array(list(vv) + [np.zeros_like(vv[0])])
+    orc.forcing = TCRandomForcing(orc.oper, NS3DStrat.keys_state_spect, 2, 4,
+                                  key_forced="vt_fft", anisotropic=STRAT_ANISO)
+    t0 = time.perf_counter()
+    orc.run(steps)
+    t_step = (time.perf_counter() - t0) / steps
+    ref = orc.state_spect
+    E_ref = _energy_shells(op, ref[:3])
+    res = {"case": (f"ns3d.strat {n}^3 forced, {steps} RK4 steps at dt = {dt:.4g}, "
+                    f"same seeded fields and forcing draws, FP64 numpy oracle")}
+    for prec, a in got.items():
+        res[f"{prec}_vs_oracle"] = {
+            "state_rel_l2": _rel_l2(a, ref),
+            "energy_spectrum": _spectrum_error(_energy_shells(op, a[:3]), E_ref, n // 3)}
+    return res, t_step, cores
+
+
+def bench_strat(args, torch, _lib, copy_gbps):
+    """ns3d.strat workload at n^3 FP32: steps/s with the state in HBM, step
+    roofline, per-pass table, FP64-oracle error at 256^3, CPU baseline."""
+    n, prec = args.n, (4 if args.precision == "float32" else 8)
+    print(f"[bench] building ns3d.strat {n}^3 {args.precision}", file=sys.stderr, flush=True)
+    with contextlib.redirect_stdout(sys.stderr):
+        sim = make_strat_sim(n, args.precision, seed=0)
+        ts = sim.time_stepping
+        ts.prepare_main_loop()
+    plan = sim.oper.plan.handle
+    for _ in range(args.warmup):
+        ts.one_time_step()
+    torch.cuda.synchronize()
+    print(f"[bench] warm-up done, dt = {ts.deltat:.4g}", file=sys.stderr, flush=True)
+    _lib.call("fsx_timing_enable", plan, 1)
+    ms8 = (_lib.ctypes.c_double * 8)()
+    by8 = (_lib.ctypes.c_double * 8)()
+    cn8 = (_lib.ctypes.c_longlong * 8)()
+    _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts.one_time_step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+    _lib.call("fsx_timing_enable", plan, 0)
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    steps_per_s = args.steps / elapsed
+    kernels = {}
+    for k in range(7):
+        if cn8[k]:
+            kernels[KINDS[k]] = {
+                "avg_ms": ms8[k] / cn8[k],
+                "launches_per_step": cn8[k] / args.steps,
+                "share_of_step": ms8[k] / (elapsed * 1e3),
+                "alg_GBps": by8[k] / (ms8[k] * 1e-3) / 1e9,
+            }
+    dom = max(range(7), key=lambda k: ms8[k])
+    achieved = by8[dom] / (ms8[dom] * 1e-3) / 1e9
+    balg = b_alg_strat(n, prec)
+    t_final = ts.t
+    del sim, ts
+    gc.collect()
+    torch.cuda.empty_cache()
+    result = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == 4 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (
+                f"ns3d.strat {n}^3 {args.precision} RK4, N = 4, Pr = 1, nu_8 with "
+                "k_max*eta = 1, toroidal tcrandom_anisotropic forcing at k in [2, 4], "
+                "isotropic E(k)~k^4 exp(-2(k/4)^2) velocity (seed 0), b = 0, CFL 0.5"
+            ),
+            "grid": [n, n, n],
+            "parallelism": "single GPU",
+            "t_after_timed_steps": t_final,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": KINDS[dom], "achieved": achieved,
+            "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": read_traffic(f"strat_{n}_{args.precision}:{KINDS[dom]}"),
+            "copy_peak_measured": copy_gbps, "frac_of_copy": achieved / copy_gbps,
+        },
+        "step_roofline": {
+            "B_alg_GB_per_step": balg / 1e9,
+            "achieved_GBps": balg * steps_per_s / 1e9,
+            "frac": balg * steps_per_s / 1e9 / PEAK_HBM_GBPS,
+            "frac_of_copy": balg * steps_per_s / 1e9 / copy_gbps,
+        },
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] FP64 oracle check and CPU baseline at "
+              f"{args.cpu_n}^3", file=sys.stderr, flush=True)
+        with contextlib.redirect_stdout(sys.stderr):
+            acc, t_step, cores = strat_oracle_check(args.cpu_n, steps=args.oracle_steps)
+        result["accuracy"] = acc
+        m = args.cpu_n
+        scale = (n**3 * math.log2(n**3)) / (m**3 * math.log2(m**3))
+        result["cpu_baseline"] = {
+            "value": 1.0 / (t_step * scale), "unit": "RK4 steps/s", "cores": cores,
+            "kind": "port",
+            "sample": (f"{args.oracle_steps} forced RK4 steps of the same workload at "
+                       f"{m}^3 FP64 with the oracle ({t_step:.2f} s/step, scipy.fft "
+                       f"workers={cores}), scaled by N log N (x{scale:.1f}) to {n}^3"),
+        }
+    print(json.dumps(result), flush=True)
 
 
 def cpu_baseline(n_sample, n_metric, steps=1):
@@ -537,6 +822,13 @@ def main():
     ap.add_argument("--closure-train-steps", type=int, default=2000)
     ap.add_argument("--closure-eval-t", type=float, default=4.0)
     ap.add_argument("--closure-sample-dt", type=float, default=0.2)
+    ap.add_argument("--strat", action="store_true",
+                    help="time the ns3d.strat forced workload (use --n 1024)")
+    ap.add_argument("--oracle-steps", type=int, default=3,
+                    help="steps of the FP64 oracle check of --strat")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-vs-FP64 accuracy runs of the ns3d bench")
+    ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first "
                          "(calibration kernel for the HBM byte counters)")
@@ -561,6 +853,8 @@ def main():
         return bench_ensemble(args, torch, _lib, copy_gbps)
     if args.closure:
         return bench_closure(args, torch, _lib)
+    if args.strat:
+        return bench_strat(args, torch, _lib, copy_gbps)
     if args.copy_mb:
         src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -570,8 +864,11 @@ def main():
         del src, dst
     print(f"[bench] building ns3d {n}^3 {args.precision}", file=sys.stderr, flush=True)
     # the solver's own progress messages go to stderr: stdout is the JSON line
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft
+
+    vv0 = ns3d_spectrum_vv_fft(n, seed=0)
     with contextlib.redirect_stdout(sys.stderr):
-        sim = make_sim(n, args.precision)
+        sim = make_sim(n, args.precision, vv=vv0)
     ts = sim.time_stepping
     plan = sim.oper.plan.handle
     print(f"[bench] dt = {ts.deltat:.6g}; warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -655,10 +952,18 @@ def main():
         "kernels": kernels,
         "diagnostics": diagnostics,
     }
+    del sim, ts
+    gc.collect()
+    torch.cuda.empty_cache()
+    if not args.no_accuracy:
+        print("[bench] accuracy: FP32 and FP64 engine vs the FP64 oracle", file=sys.stderr,
+              flush=True)
+        with contextlib.redirect_stdout(sys.stderr):
+            result["accuracy"] = ns3d_accuracy(torch, n, args.accuracy_steps, vv=vv0)
+        del vv0
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        del sim
-        torch.cuda.empty_cache()
         result["cpu_baseline"] = cpu_baseline(args.cpu_n, n)
     print(json.dumps(result), flush=True)
 

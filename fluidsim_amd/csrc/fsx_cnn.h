@@ -116,7 +116,9 @@ __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
 // Epilogue: FUSE = 0 writes the FP32 pre-activations y (co_out channels);
 // FUSE = 1 writes the BF16 GELU activations a[p][32] of a hidden layer
 // instead (co_out = 32); FUSE = 2 writes both.
-template <int CI, int FUSE = 0>
+// XF32 (CI = 8 only): x is one FP32 channel x[B][H][W], rounded to BF16 and
+// zero-padded to 8 channels while it is staged.
+template <int CI, int FUSE = 0, bool XF32 = false>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
                                                   const uint16_t* __restrict__ w,
                                                   const float* __restrict__ bias,
@@ -158,7 +160,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x
         int gy = y0 - 2 + pr, gx = pc - 2;
         gy += gy < 0 ? H : (gy >= H ? -H : 0);
         gx += gx < 0 ? W : (gx >= W ? -W : 0);
-        vp[j] = *reinterpret_cast<const u32x4*>(&x[(((size_t)b * H + gy) * W + gx) * CI + 8 * q]);
+        if constexpr (XF32) {
+          static_assert(CI == 8, "one FP32 input channel pads to 8");
+          const float v = reinterpret_cast<const float*>(x)[((size_t)b * H + gy) * W + gx];
+          vp[j] = u32x4{to_bf16(v), 0u, 0u, 0u};
+        } else {
+          vp[j] = *reinterpret_cast<const u32x4*>(
+              &x[(((size_t)b * H + gy) * W + gx) * CI + 8 * q]);
+        }
       }
       pc += cstep;
       while (pc >= WP) {

@@ -2178,21 +2178,25 @@ int cnn_grid(long long n) {
 
 // fuse: 0 -> FP32 pre-activations y; 1 -> BF16 GELU activations a (32
 // channels, co_out = 32); 2 -> both
-template <int CI, int FUSE>
+template <int CI, int FUSE, bool XF32>
 void conv_fwd_launch(unsigned nblk, size_t lds, hipStream_t s, const uint16_t* x,
                      const uint16_t* w, const float* bias, float* y, int H, int W, int co_out,
                      int R, uint16_t* a) {
-  hipLaunchKernelGGL((k_conv_fwd<CI, FUSE>), dim3(nblk), dim3(256), lds, s, x, w, bias, y, H, W,
-                     co_out, R, a);
+  hipLaunchKernelGGL((k_conv_fwd<CI, FUSE, XF32>), dim3(nblk), dim3(256), lds, s, x, w, bias, y,
+                     H, W, co_out, R, a);
 }
 
+// x_f32: x is one FP32 channel (ci = 8 layer: padded while staged)
 int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, const float* bias,
-                 int co_out, float* y, hipStream_t s, int fuse = 0, void* a = nullptr) {
+                 int co_out, float* y, hipStream_t s, int fuse = 0, void* a = nullptr,
+                 bool x_f32 = false) {
   if (!x || !w || !bias || B < 1 || fuse < 0 || fuse > 2) return fail(FSX_ERR_ARG, "bad argument");
   if ((fuse != 1 && !y) || (fuse != 0 && !a)) return fail(FSX_ERR_ARG, "bad argument");
   if (!cnn_channels_ok(ci)) return fail(FSX_ERR_ARG, "input channels must be 8 or 32");
   if (co_out < 1 || co_out > 32) return fail(FSX_ERR_ARG, "output channels must be 1..32");
   if (fuse && co_out != 32) return fail(FSX_ERR_ARG, "fused activations need 32 output channels");
+  if (x_f32 && (ci != 8 || fuse != 1))
+    return fail(FSX_ERR_ARG, "an FP32 input is one channel (padded to 8) of a hidden layer");
   if (H < kCnnRows || H % kCnnRows || W < 32 || W % 32)
     return fail(FSX_ERR_ARG, "H must be a multiple of 4 and W a multiple of 32");
   // rows per workgroup: 4 when there are enough workgroups, fewer for
@@ -2206,18 +2210,19 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   const auto* wi = reinterpret_cast<const uint16_t*>(w);
   auto* ai = reinterpret_cast<uint16_t*>(a);
   int rc;
-#define FSX_CONV_FWD(CI_, F_)                                                           \
-  if (ci == CI_ && fuse == F_) {                                                        \
-    if ((rc = set_lds(k_conv_fwd<CI_, F_>, conv_fwd_lds(CI_, W, kCnnRows)))) return rc; \
-    if ((rc = tm_pre(s))) return rc;                                                    \
-    conv_fwd_launch<CI_, F_>(nblk, lds, s, xi, wi, bias, y, H, W, co_out, R, ai);       \
+#define FSX_CONV_FWD(CI_, F_, XF_)                                                      \
+  if (ci == CI_ && fuse == F_ && x_f32 == XF_) {                                          \
+    if ((rc = set_lds(k_conv_fwd<CI_, F_, XF_>, conv_fwd_lds(CI_, W, kCnnRows)))) return rc; \
+    if ((rc = tm_pre(s))) return rc;                                                      \
+    conv_fwd_launch<CI_, F_, XF_>(nblk, lds, s, xi, wi, bias, y, H, W, co_out, R, ai);    \
   }
-  FSX_CONV_FWD(8, 0)
-  FSX_CONV_FWD(8, 1)
-  FSX_CONV_FWD(8, 2)
-  FSX_CONV_FWD(32, 0)
-  FSX_CONV_FWD(32, 1)
-  FSX_CONV_FWD(32, 2)
+  FSX_CONV_FWD(8, 0, false)
+  FSX_CONV_FWD(8, 1, false)
+  FSX_CONV_FWD(8, 2, false)
+  FSX_CONV_FWD(8, 1, true)
+  FSX_CONV_FWD(32, 0, false)
+  FSX_CONV_FWD(32, 1, false)
+  FSX_CONV_FWD(32, 2, false)
 #undef FSX_CONV_FWD
   FSX_HIP(hipGetLastError());
   return tm_post(s);
@@ -2238,9 +2243,10 @@ long long cnn_forward_work(int B, int H, int W) { return 2LL * B * H * W * 32 * 
 // Network forward: hidden layers write their BF16 GELU activations from the
 // convolution epilogue (ping-pong between the two buffers of work), the last
 // layer writes its FP32 outputs to y.
+// x_f32: x is one FP32 channel (the first layer pads it to 8 while staging)
 int cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
                 const float* const* b, const void* x, int B, int H, int W, float* y,
-                void* work, hipStream_t s) {
+                void* work, hipStream_t s, bool x_f32 = false) {
   const long long npix = (long long)B * H * W;
   uint16_t* act[2] = {reinterpret_cast<uint16_t*>(work),
                       reinterpret_cast<uint16_t*>(work) + npix * 32};
@@ -2255,9 +2261,10 @@ int cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
       note(K_CNN, 2.0 * npix * 32 * kCnnTaps * 32);
     else
       note(K_OTHER, 0.0);
+    if (x_f32 && l == 0 && last) return fail(FSX_ERR_ARG, "an FP32 input needs a hidden first layer");
     const int rc = last ? cnn_conv_fwd(in, B, H, W, cin[l], w[l], b[l], cout[l], y, s)
                         : cnn_conv_fwd(in, B, H, W, cin[l], w[l], b[l], 32, nullptr, s, 1,
-                                       act[l & 1]);
+                                       act[l & 1], x_f32 && l == 0);
     if (rc) return rc;
     in = act[l & 1];
   }
@@ -2265,10 +2272,10 @@ int cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
 }
 
 // workspace of fsx_ns2d_closure_term: vorticity and network output (FP32),
-// the BF16 network input (8 channels), the network's activations
+// the network's activations
 long long closure_work(const fsx_plan_s* p) {
   const long long npix = (long long)p->ny * p->nx;
-  return npix * (4 + 4 + 16) + cnn_forward_work(1, p->ny, p->nx);
+  return npix * (4 + 4) + cnn_forward_work(1, p->ny, p->nx);
 }
 
 }  // namespace
@@ -2443,11 +2450,9 @@ int fsx_ns2d_closure_term(fsx_plan p, int nl, const int* cin, const int* cout,
   char* wb = reinterpret_cast<char*>(work);
   float* X = reinterpret_cast<float*>(wb);
   float* Y = reinterpret_cast<float*>(wb + npix * 4);
-  void* X8 = wb + npix * 8;
-  void* fw = wb + npix * 24;
+  void* fw = wb + npix * 8;
   int rc = do_ifft<float>(p, rot_fft, X, 1, s);
-  if (!rc) rc = cnn_act(X, npix, 1, 8, 0, X8, s);
-  if (!rc) rc = cnn_forward(nl, cin, cout, w, b, X8, 1, p->ny, p->nx, Y, fw, s);
+  if (!rc) rc = cnn_forward(nl, cin, cout, w, b, X, 1, p->ny, p->nx, Y, fw, s, true);
   if (!rc) rc = do_fft<float>(p, Y, out_fft, 1, s);
   if (!rc) rc = fsx_dealias(p, out_fft, 1, 0, stream);
   return rc;

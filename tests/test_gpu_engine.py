@@ -29,7 +29,8 @@ def S():
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("shape", [(16, 16), (32, 64), (256, 256), (8, 512), (2048, 16)])
+@pytest.mark.parametrize("shape", [(16, 16), (32, 64), (256, 256), (8, 512), (2048, 16),
+                                   (8, 1024), (4, 2048), (1024, 8)])
 def test_fft_ifft_2d(prec, shape):
     ny, nx = shape
     print(f"case 2D fft prec={prec} shape={shape}", flush=True)
@@ -54,7 +55,8 @@ def test_fft_ifft_2d(prec, shape):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 32), (64, 64, 64), (4, 32, 128)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 32), (64, 64, 64), (4, 32, 128),
+                                   (4, 8, 1024), (1024, 4, 8), (8, 1024, 4)])
 def test_fft_ifft_3d(prec, shape):
     nz, ny, nx = shape
     print(f"case 3D fft prec={prec} shape={shape}", flush=True)
@@ -82,7 +84,7 @@ def _ns2d_case(n, seed=0):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("n", [32, 256])
+@pytest.mark.parametrize("n", [32, 256, 1024, 2048])
 def test_ns2d_tendency(prec, n):
     print(f"case ns2d tendency prec={prec} n={n}", flush=True)
     oper, sim = _ns2d_case(n)
@@ -111,7 +113,8 @@ def _ns3d_case(shape, seed=0):
 
 
 @pytest.mark.parametrize("prec", [8, 4])
-@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32), (64, 64, 64), (4, 8, 1024),
+                                   (1024, 8, 4), (8, 1024, 8)])
 def test_ns3d_tendency(prec, shape):
     print(f"case ns3d tendency prec={prec} shape={shape}", flush=True)
     oper, sim = _ns3d_case(shape)
