@@ -219,6 +219,25 @@ int set_lds(K kern, size_t lds) {
   return 0;
 }
 
+template <typename T, int DIR, int E, int KT, class Op>
+int launch_strided_tile(Op& op, int N, long long blocks, int nfb, hipStream_t s) {
+  const int tile = E * KT;
+  const size_t lds = ((size_t)tile + tile / 32 + N) * sizeof(cpx<T>);
+  auto kern = k_strided<T, DIR, E, KT, Op>;
+  static size_t lds_set = 0;
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
+  if (lds > lds_set) {
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
+    lds_set = lds;
+  }
+  int rc = tm_pre(s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks, (unsigned)nfb), dim3(KT), lds, s, op);
+  FSX_HIP(hipGetLastError());
+  return tm_post(s);
+}
+
 template <typename T, int DIR, class Op>
 int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw,
                    hipStream_t s, bool kio = false, int lines_cap = 0) {
@@ -226,7 +245,13 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   const int tile = E * kThr;
   if (N > tile) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
   const int nfb = Op::kFieldPerBlock ? nf : 1;
-  int L = tile / N;
+  // FP32 transforms that leave fewer than 8 lines in a tile of kThr threads
+  // run 2 kThr threads per workgroup (twice the lines, longer contiguous
+  // runs per position) when that tile's LDS fits
+  constexpr int KW = 2 * kThr;
+  const bool wide_ok = sizeof(T) == 4 && tile / N < 8 &&
+                       ((size_t)E * KW + E * KW / 32 + N) * sizeof(cpx<T>) <= kLdsMax;
+  int L = (wide_ok ? E * KW : tile) / N;
   int cap = 1;
   while (cap < nlines) cap <<= 1;
   if (L > cap) L = cap;
@@ -237,6 +262,7 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
     // enough workgroups to fill the chip when the problem is small
     while (L > 1 && (long long)nfb * nouter * ((nlines + L - 1) / L) < kMinBlocks) L >>= 1;
   }
+  const bool wide = wide_ok && L > tile / N;  // a kThr tile cannot hold L lines
   op.d.N = N;
   op.d.logN = ilog2(N);
   op.d.L = L;
@@ -245,26 +271,15 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   op.d.nlines = nlines;
   op.d.nf = nf;
   op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
-  op.d.kio = kio;
+  // tiles whose runs are shorter than a 128-byte cache line share lines with
+  // their neighbours: keep neighbours on one XCD as for K rows
+  op.d.kio = kio || (size_t)L * sizeof(cpx<T>) < 128;
   if (nouter <= 0 || nlines <= 0 || nf <= 0) return 0;
   const long long nchunks = (nlines + L - 1) / L;
   const long long blocks = nchunks * nouter;
   if (blocks > 0x7fffffffLL) return fail(FSX_ERR_ARG, "grid too large");
-  const size_t lds = ((size_t)tile + tile / 32 + N) * sizeof(cpx<T>);
-  auto kern = k_strided<T, DIR, E, kThr, Op>;
-  static size_t lds_set = 0;
-  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
-  if (lds > lds_set) {
-    int rc = set_lds(kern, lds);
-    if (rc) return rc;
-    lds_set = lds;
-  }
-  int rc = tm_pre(s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks, (unsigned)nfb), dim3(kThr), lds,
-                     s, op);
-  FSX_HIP(hipGetLastError());
-  return tm_post(s);
+  return wide ? launch_strided_tile<T, DIR, E, KW>(op, N, blocks, nfb, s)
+              : launch_strided_tile<T, DIR, E, kThr>(op, N, blocks, nfb, s);
 }
 
 template <typename T, class Op>

@@ -32,7 +32,7 @@ struct StridedDesc {
   int nouter;     // number of independent planes
   int nlines;     // lines per plane
   int nf;         // fields transformed one after the other
-  int kio;        // the pass reads or writes K rows (tiles remapped per XCD)
+  int kio;        // tiles remapped per XCD (K rows, or runs under a cache line)
   const cpx<T>* tw;  // exp(-2 pi i m / N), m < N (copied to LDS)
 };
 
@@ -167,9 +167,11 @@ __device__ __forceinline__ bool last_slot(int e, int logN, int logL, int& line,
 // ---------------------------------------------------------------- strided
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
 // XCD b mod FSX_NXCD), each XCD with its own L2.  In passes over K rows
-// (nkx = nx/2 + 1 complex values, not line-aligned) neighbouring line chunks
-// share the cache lines that straddle their boundary, so there consecutive
-// tiles are given to the same XCD: XCD x takes the x-th contiguous range.
+// (nkx = nx/2 + 1 complex values, not line-aligned), and in passes whose
+// tiles hold runs shorter than a 128-byte line (long transforms leave few
+// lines per tile), neighbouring line chunks share cache lines, so there
+// consecutive tiles are given to the same XCD: XCD x takes the x-th
+// contiguous range.
 #ifndef FSX_NXCD
 #define FSX_NXCD 8
 #endif
