@@ -67,8 +67,10 @@ def dispatch_values(db, counter):
 def traffic(fetch_db, write_db, copy_mib, prefix=""):
     """HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
     each scaled by what the counter reads for the device-to-device copy of
-    COPY_MIB MiB (the copyBuffer dispatch that reads and writes COPY_MIB MiB):
-    factor = true KiB / counted KiB."""
+    COPY_MIB MiB: factor = true KiB / counted KiB. The calibration copy is
+    the first copyBuffer dispatch that counts at least 1 MiB (bench.py
+    times its 1 GiB device copy before anything else runs; later copies,
+    e.g. of whole state fields, have other sizes)."""
     import json
 
     true_kib = copy_mib * 1024.0
@@ -79,7 +81,10 @@ def traffic(fetch_db, write_db, copy_mib, prefix=""):
         copies = [r for r in rows if "copyBuffer" in r[0]]
         if not copies:
             raise SystemExit(f"no copyBuffer dispatch in {db}")
-        cal = max(copies, key=lambda r: r[1])  # the large calibration copy
+        large = [r for r in copies if r[2] >= 1024]
+        if not large:
+            raise SystemExit(f"no calibration copy in {db}")
+        cal = large[0]
         factors[counter] = true_kib / cal[2]
         agg = defaultdict(list)
         for name, dur, val, grid in rows:
