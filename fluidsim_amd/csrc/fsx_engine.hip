@@ -245,11 +245,14 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   const int tile = E * kThr;
   if (N > tile) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
   const int nfb = Op::kFieldPerBlock ? nf : 1;
-  // FP32 transforms that leave fewer than 8 lines in a tile of kThr threads
-  // run 2 kThr threads per workgroup (twice the lines, longer contiguous
-  // runs per position) when that tile's LDS fits
+  // FP32 transforms whose tile of kThr threads holds short runs per
+  // position run 2 kThr threads per workgroup (twice the lines) when that
+  // tile's LDS fits: the multi-transform z-passes (halved tiles) when runs
+  // are shorter than a 128-byte line, plain passes below 8 lines (measured:
+  // wider plain y-passes at 8 lines are slower)
   constexpr int KW = 2 * kThr;
-  const bool wide_ok = sizeof(T) == 4 && tile / N < 8 &&
+  constexpr int wide_below = Op::kEptDiv > 1 ? (int)(128 / sizeof(cpx<T>)) : 8;
+  const bool wide_ok = sizeof(T) == 4 && tile / N < wide_below &&
                        ((size_t)E * KW + E * KW / 32 + N) * sizeof(cpx<T>) <= kLdsMax;
   int L = (wide_ok ? E * KW : tile) / N;
   int cap = 1;
