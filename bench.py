@@ -795,7 +795,59 @@ def bench_closure(args, torch, _lib):
         "timings_s": {"training_data": t_data, "training": t_train},
         "training_loss": {"first": l1[0][1], "last": l2[-1][1]},
     }
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle)")
+        result["cpu_baseline"] = cpu_baseline_closure(net.folded(sw, s_s), nd, nl, nu)
     print(json.dumps(result), flush=True)
+
+
+def cpu_baseline_closure(net, nd, nl, nu, reps=2):
+    """FP64 oracle on the host cores: one LES RK4 step with the network at
+    nl^2 (the ns2d step plus 4 network forwards, numpy) and one DNS RK4 step
+    at nd^2, each averaged over `reps`."""
+    from oracle import cnn as C
+    from oracle import operators as O
+    from oracle.solvers import NS2D
+    from fluidsim_amd.synthetic import ns2d_noise_rot_fft
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    O.set_workers(cores)
+    layers = []
+    for l in range(net.nlayers):
+        w = net.w[l].cpu().numpy().astype(np.float64)
+        co = net.cout[l]
+        layers.append((w[:co], net.b[l].cpu().numpy()[:co].astype(np.float64)))
+    onet = C.Net(layers)
+
+    def step_time(n):
+        sim = NS2D(O.Oper2D(n, n), nu_2=nu, deltat0=1e-3)
+        sim.init_from_rotfft(ns2d_noise_rot_fft(n, n, seed=0, kmax=12))
+        sim.one_time_step_computation()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sim.one_time_step_computation()
+        return (time.perf_counter() - t0) / reps
+
+    t_les = step_time(nl)
+    x = np.zeros((1, nl, nl, 8))
+    x[0, :, :, 0] = np.random.default_rng(0).standard_normal((nl, nl))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        onet.forward(x)
+    t_net = (time.perf_counter() - t0) / reps
+    t_dns = step_time(nd)
+    t_les_net = t_les + 4 * t_net
+    return {
+        "value": 1.0 / t_les_net,
+        "unit": f"LES RK4 steps/s ({nl}^2 with the network)",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"FP64 oracle, {reps} steps each: ns2d {nl}^2 RK4 step {t_les * 1e3:.1f} ms "
+                   f"+ 4 network forwards of {t_net * 1e3:.1f} ms (numpy) = "
+                   f"{t_les_net * 1e3:.1f} ms per LES step; DNS {nd}^2 RK4 step "
+                   f"{t_dns * 1e3:.1f} ms (scipy.fft workers={cores})"),
+        "dns_steps_per_s": 1.0 / t_dns,
+    }
 
 
 def read_traffic(kernel_key):
