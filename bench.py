@@ -271,6 +271,77 @@ def strat_oracle_check(n=256, steps=3, seed=0):
     return res, t_step, cores
 
 
+def bench_ns2d(args, torch, _lib, copy_gbps):
+    """ns2d at n^2 (the DNS of the closure workload at 2048^2 FP32): forced
+    (tcrandom at k in [8, 10]), nu_2 = 1e-5, CFL 0.5; steps/s with the state
+    in HBM, step roofline (B_alg = 21 (b_X + b_K) + 17 b_K), per-pass table."""
+    from fluidsim_amd import closure as CL
+    from fluidsim_amd.synthetic import ns2d_noise_rot_fft
+
+    n, prec = args.n, (4 if args.precision == "float32" else 8)
+    Cls, params = CL.ns2d_params(n, 1e-5, precision=args.precision)
+    np.random.seed(0)
+    with contextlib.redirect_stdout(sys.stderr):
+        sim = Cls(params)
+        sim.state.init_statespect_from(
+            rot_fft=ns2d_noise_rot_fft(n, n, seed=0, kmax=12, velo_max=0.5))
+        ts = sim.time_stepping
+        ts.prepare_main_loop()
+    plan = sim.oper.plan.handle
+    for _ in range(args.warmup):
+        ts.one_time_step()
+    torch.cuda.synchronize()
+    _lib.call("fsx_timing_enable", plan, 1)
+    ms8 = (_lib.ctypes.c_double * 8)()
+    by8 = (_lib.ctypes.c_double * 8)()
+    cn8 = (_lib.ctypes.c_longlong * 8)()
+    _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+    _lib.call("fsx_timing_enable", plan, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts.one_time_step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # per-pass durations from a second, event-timed run of the same steps
+    _lib.call("fsx_timing_enable", plan, 1)
+    for _ in range(args.steps):
+        ts.one_time_step()
+    _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
+    _lib.call("fsx_timing_enable", plan, 0)
+    steps_per_s = args.steps / elapsed
+    kernels = {}
+    for k in range(7):
+        if cn8[k]:
+            kernels[KINDS[k]] = {
+                "avg_ms": ms8[k] / cn8[k],
+                "launches_per_step": cn8[k] / args.steps,
+                "alg_GBps": by8[k] / (ms8[k] * 1e-3) / 1e9,
+            }
+    dom = max(range(7), key=lambda k: ms8[k])
+    achieved = by8[dom] / (ms8[dom] * 1e-3) / 1e9
+    balg = b_alg_ns2d(n, prec)
+    result = {
+        "metric": METRIC, "value": steps_per_s, "unit": "RK4 steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if prec == 4 else "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"ns2d {n}^2 {args.precision} RK4, nu_2 = 1e-5, tcrandom "
+                                "forcing at k in [8, 10], CFL 0.5 (the closure workload's "
+                                "DNS)"),
+                   "grid": [n, n], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": KINDS[dom], "achieved": achieved,
+                     "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS,
+                     "traffic": None, "copy_peak_measured": copy_gbps},
+        "step_roofline": {"B_alg_GB_per_step": balg / 1e9,
+                          "achieved_GBps": balg * steps_per_s / 1e9,
+                          "frac": balg * steps_per_s / 1e9 / PEAK_HBM_GBPS},
+        "kernels": kernels,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def bench_strat(args, torch, _lib, copy_gbps):
     """ns3d.strat workload at n^3 FP32: steps/s with the state in HBM, step
     roofline, per-pass table, FP64-oracle error at 256^3, CPU baseline."""
@@ -881,6 +952,8 @@ def main():
     ap.add_argument("--closure-train-steps", type=int, default=2000)
     ap.add_argument("--closure-eval-t", type=float, default=4.0)
     ap.add_argument("--closure-sample-dt", type=float, default=0.2)
+    ap.add_argument("--ns2d", action="store_true",
+                    help="time the forced ns2d DNS of the closure workload (use --n 2048)")
     ap.add_argument("--strat", action="store_true",
                     help="time the ns3d.strat forced workload (use --n 1024)")
     ap.add_argument("--oracle-steps", type=int, default=3,
@@ -914,6 +987,8 @@ def main():
         return bench_closure(args, torch, _lib)
     if args.strat:
         return bench_strat(args, torch, _lib, copy_gbps)
+    if args.ns2d:
+        return bench_ns2d(args, torch, _lib, copy_gbps)
     if args.copy_mb:
         src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

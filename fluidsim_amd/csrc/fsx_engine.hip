@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -181,7 +182,10 @@ constexpr int kThr = 512;   // threads per workgroup of the strided passes
 constexpr int kXThr = FSX_XPASS_THREADS;  // threads per workgroup of the row pass
 constexpr int kEPT32 = 16;  // 8192 complex64 per tile  = 64 KiB of LDS
 constexpr int kEPT64 = 8;   // 4096 complex128 per tile = 64 KiB of LDS
-constexpr int kMinBlocks = 1024;  // shrink tiles of small problems below this
+#ifndef FSX_MIN_BLOCKS
+#define FSX_MIN_BLOCKS 1024
+#endif
+constexpr int kMinBlocks = FSX_MIN_BLOCKS;  // shrink tiles of small problems below this
 
 template <typename T>
 constexpr int ept() {
@@ -219,10 +223,22 @@ int set_lds(K kern, size_t lds) {
   return 0;
 }
 
+// Lines per tile an op may transform at once, as a multiple of L (an op
+// with several outputs per line can stack them as extra lines).
+template <class Op, class = void>
+struct line_mul { static constexpr int value = 1; };
+template <class Op>
+struct line_mul<Op, std::void_t<decltype(Op::kLineMul)>> {
+  static constexpr int value = Op::kLineMul;
+};
+
+// LDS of a strided tile of `tile` elements (N positions x its lines; the
+// kernel touches only the padded tile after the N twiddles)
 template <typename T, int DIR, int E, int KT, class Op>
-int launch_strided_tile(Op& op, int N, long long blocks, int nfb, hipStream_t s) {
-  const int tile = E * KT;
-  const size_t lds = ((size_t)tile + tile / 32 + N) * sizeof(cpx<T>);
+int launch_strided_tile(Op& op, int N, size_t tile_max, long long blocks, int nfb,
+                        hipStream_t s) {
+  const size_t tile = std::min(tile_max, (size_t)E * KT);
+  const size_t lds = (tile + tile / 32 + N) * sizeof(cpx<T>);
   auto kern = k_strided<T, DIR, E, KT, Op>;
   static size_t lds_set = 0;
   if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
@@ -262,8 +278,12 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
     // tuned lines per tile (power of two)
     while (L > lines_cap && L > 1) L >>= 1;
   } else {
-    // enough workgroups to fill the chip when the problem is small
-    while (L > 1 && (long long)nfb * nouter * ((nlines + L - 1) / L) < kMinBlocks) L >>= 1;
+    // enough workgroups to fill the chip when the problem is small; long
+    // transforms keep more lines per tile (measured on the 2D y-passes:
+    // at length 2048, 4 lines in 171 workgroups beat 1 line in 683; at
+    // length 1024, 1 line in 342 beats 2 in 171)
+    const int min_blocks = N >= 2048 ? 128 : kMinBlocks;
+    while (L > 1 && (long long)nfb * nouter * ((nlines + L - 1) / L) < min_blocks) L >>= 1;
   }
   const bool wide = wide_ok && L > tile / N;  // a kThr tile cannot hold L lines
   op.d.N = N;
@@ -281,8 +301,17 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   const long long nchunks = (nlines + L - 1) / L;
   const long long blocks = nchunks * nouter;
   if (blocks > 0x7fffffffLL) return fail(FSX_ERR_ARG, "grid too large");
-  return wide ? launch_strided_tile<T, DIR, E, KW>(op, N, blocks, nfb, s)
-              : launch_strided_tile<T, DIR, E, kThr>(op, N, blocks, nfb, s);
+  // elements the op transforms at once: N L, or N L times its line
+  // multiple when that fits a kThr tile
+  size_t need = (size_t)N * L;
+  if (!wide && need * line_mul<Op>::value <= (size_t)E * kThr) need *= line_mul<Op>::value;
+  if (wide) return launch_strided_tile<T, DIR, E, KW>(op, N, need, blocks, nfb, s);
+  // small problems (few lines per tile): as many threads as the tile needs
+  if ((size_t)E * kThr / 4 >= need)
+    return launch_strided_tile<T, DIR, E, kThr / 4>(op, N, need, blocks, nfb, s);
+  if ((size_t)E * kThr / 2 >= need)
+    return launch_strided_tile<T, DIR, E, kThr / 2>(op, N, need, blocks, nfb, s);
+  return launch_strided_tile<T, DIR, E, kThr>(op, N, need, blocks, nfb, s);
 }
 
 template <typename T, class Op>
@@ -418,7 +447,7 @@ int launch_xrow(int nx, int nrows, int mx_in, int ld_in, int mx_out, int ld_out,
     case 4: return launch_xrow_m<T, LOGP + 4>(d, op, s);
     case 5: return launch_xrow_m<T, LOGP + 5>(d, op, s);
     case 6: return launch_xrow_m<T, LOGP + 6>(d, op, s);
-    default: return 1;
+    default: return 1;  // longer rows: the LDS row pass (k_xpass)
   }
 }
 

@@ -122,6 +122,7 @@ __device__ __forceinline__ void dft8(cpx<T>* v) {
 
 template <typename T, int DIR, int R>
 __device__ __forceinline__ void dftR(cpx<T>* v) {
+  static_assert(R == 8 || R == 4 || R == 2, "radix 2, 4 or 8");
   if constexpr (R == 8)
     dft8<T, DIR>(v);
   else if constexpr (R == 4)

@@ -449,6 +449,8 @@ template <typename T>
 struct OpYInv2NS {
   static constexpr int kEptDiv = 1;
   static constexpr bool kFieldPerBlock = false;
+  // the four outputs can share one tile as 4 L lines (small transforms)
+  static constexpr int kLineMul = 4;
   StridedDesc<T> d;
   const cpx<T>* in;
   cpx<T>* out;
@@ -475,6 +477,31 @@ struct OpYInv2NS {
       if (k2 == T(0)) k2 = T(1e-14);
       return T(1) / k2;
     };
+    if (((size_t)d.N << (d.logL + 2)) <= (size_t)EPT * kT) {
+      // one transform of 4 L lines: line l is output l >> logL of line
+      // l & (L - 1), each line computed as in the sequential form below
+      const int lm = (1 << d.logL) - 1;
+      tile_fft<T, DIR, EPT, kT>(
+          buf, tw, d.logN, d.logL + 2,
+          [&](int, int l, int p) {
+            const int c = l >> d.logL, li = l & lm;
+            const int line = line0 + li;
+            cpx<T> r = zero<T>();
+            if (line < nlines && ym.compact(p) >= 0) r = in[p * nkx + line];
+            T f;
+            if (c == 0)
+              f = T(ksigned(p, ny)) * dky * inv_k2(li, p);
+            else if (c == 1)
+              f = -(T(line0 + li) * dkx) * inv_k2(li, p);
+            else if (c == 2)
+              f = T(line0 + li) * dkx;
+            else
+              f = T(ksigned(p, ny)) * dky;
+            return mul_i(scale(r, f));
+          },
+          [&](int, int l, int p, cpx<T> v) { st(l >> d.logL, l & lm, p, v); });
+      return;
+    }
     tile_fft<T, DIR, EPT, kT>(
         buf, tw, d.logN, d.logL,
         [&](int e, int l, int p) {
