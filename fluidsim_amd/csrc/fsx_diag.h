@@ -98,6 +98,11 @@ struct DiagGeom {
   int q0, nq;        // quantity group handled by this launch
   // offsets (doubles) in a block record, and its size
   int off_kx, off_ky, off_k3, off_kh, off_sum, rec;
+  // the shell and kh histograms ([off_k3, off_sum) of the record) have
+  // ncopy copies in LDS: lane l adds into copy l mod ncopy (neighbouring
+  // lanes, often in one bin, then hit different addresses); copies 1.. sit
+  // after the record and are summed into it before it is written
+  int ncopy;
 };
 
 struct DiagIn {
@@ -333,7 +338,9 @@ __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
 #pragma clang fp contract(off)
   constexpr int Q = diag_nq(KIND);
   extern __shared__ double lds[];
-  for (int t = threadIdx.x; t < g.rec; t += blockDim.x) lds[t] = 0.0;
+  const int span = g.off_sum - g.off_k3;
+  const int lds_n = g.rec + (g.ncopy > 1 ? (g.ncopy - 1) * span : 0);
+  for (int t = threadIdx.x; t < lds_n; t += blockDim.x) lds[t] = 0.0;
   __syncthreads();
   const int blk = blockIdx.x;
   const int iz = blk / g.nchunk, ch = blk % g.nchunk;
@@ -342,6 +349,9 @@ __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nw = blockDim.x >> 6;
   const bool even = (g.nx & 1) == 0;
+  // this lane's copy of the shell / kh histograms: record offset + shift
+  const int cp = g.ncopy > 1 ? lane % g.ncopy : 0;
+  const int hshift = cp == 0 ? 0 : g.rec + (cp - 1) * span - g.off_k3;
   Mode m;
   m.ikz = g.dim == 3 ? ksigned(iz, g.nz) : 0;
   m.kz = m.ikz * g.dkz;
@@ -380,7 +390,7 @@ __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
 #pragma unroll
         for (int s = 0; s < Q; ++s)
           if (s >= g.q0 && s < g.q0 + g.nq) {
-            double* h = &lds[g.off_k3 + (s - g.q0) * g.nk3];
+            double* h = &lds[hshift + g.off_k3 + (s - g.q0) * g.nk3];
             atomicAdd(h + j, (1.0 - c) * q[s]);
             if (c != 0.0) atomicAdd(h + j + 1, c * q[s]);
           }
@@ -392,7 +402,7 @@ __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
 #pragma unroll
         for (int s = 0; s < Q; ++s)
           if (s >= g.q0 && s < g.q0 + g.nq) {
-            double* h = &lds[g.off_kh + (s - g.q0) * g.nkh];
+            double* h = &lds[hshift + g.off_kh + (s - g.q0) * g.nkh];
             atomicAdd(h + j, (1.0 - c) * q[s]);
             if (c != 0.0) atomicAdd(h + j + 1, c * q[s]);
           }
@@ -417,6 +427,14 @@ __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
       if (lane == 0) atomicAdd(&lds[g.off_sum + (s - g.q0)], v);
     }
   __syncthreads();
+  if (g.ncopy > 1) {
+    for (int t = threadIdx.x; t < span; t += blockDim.x) {
+      double v = lds[g.off_k3 + t];
+      for (int c = 1; c < g.ncopy; ++c) v += lds[g.rec + (c - 1) * span + t];
+      lds[g.off_k3 + t] = v;
+    }
+    __syncthreads();
+  }
   double* dst = partial + (size_t)blk * g.rec;
   for (int t = threadIdx.x; t < g.rec; t += blockDim.x) dst[t] = lds[t];
 }

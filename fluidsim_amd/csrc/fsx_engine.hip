@@ -1877,6 +1877,13 @@ namespace {
 // LDS of one block record is kept within 64 KiB (two blocks per CU)
 constexpr int kDiagLdsDoubles = 8192;
 
+#ifndef FSX_DIAG_COPIES
+#define FSX_DIAG_COPIES 4
+#endif
+#ifndef FSX_DIAG_LDS
+#define FSX_DIAG_LDS (80 * 1024)
+#endif
+
 int diag_geom(const fsx_plan_s* p, int kind, int flags, DiagGeom& g, DiagOut& o,
               long long& total) {
   if (kind < 0 || kind >= DIAG_NKINDS) return fail(FSX_ERR_ARG, "unknown diagnostics kind");
@@ -1949,7 +1956,21 @@ int diag_record(DiagGeom& g, int q0, int nq) {
   g.off_kh = g.off_k3 + ((g.flags & DIAG_SHELL) ? nq * g.nk3 : 0);
   g.off_sum = g.off_kh + ((g.flags & DIAG_KZKH) ? nq * g.nkh : 0);
   g.rec = g.off_sum + nq;
+  // copies of the shell / kh histograms while the block's LDS stays within
+  // FSX_DIAG_LDS bytes (two blocks per CU at the default)
+  const int span = g.off_sum - g.off_k3;
+  g.ncopy = 1;
+  if (span > 0)
+    for (int c = FSX_DIAG_COPIES; c > 1; c >>= 1)
+      if ((size_t)(g.rec + (c - 1) * span) * sizeof(double) <= FSX_DIAG_LDS) {
+        g.ncopy = c;
+        break;
+      }
   return g.rec;
+}
+
+size_t diag_lds(const DiagGeom& g) {
+  return (size_t)(g.rec + (g.ncopy - 1) * (g.off_sum - g.off_k3)) * sizeof(double);
 }
 
 int diag_per_quantity(const DiagGeom& g) {
@@ -1966,10 +1987,16 @@ int diag_launch(const DiagIn& in, const Box& box, const DiagGeom& g, double* par
   for (int f = 0; f < 8; ++f) nin += in.f[f] != nullptr;
   note(K_DIAG, (double)nin * g.nz * g.ny * g.nkx * sizeof(cpx<T>) +
                    (double)nb * g.rec * sizeof(double));
+  const size_t lds = diag_lds(g);
+  static size_t lds_set = 0;
+  if (lds > lds_set) {
+    int rc = set_lds(k_diag<T, KIND>, lds);
+    if (rc) return rc;
+    lds_set = lds;
+  }
   int rc = tm_pre(s);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_diag<T, KIND>), dim3(nb), dim3(256), g.rec * sizeof(double), s, in,
-                     box, g, part);
+  hipLaunchKernelGGL((k_diag<T, KIND>), dim3(nb), dim3(256), lds, s, in, box, g, part);
   FSX_HIP(hipGetLastError());
   return tm_post(s);
 }
