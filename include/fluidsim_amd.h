@@ -373,7 +373,8 @@ int fsx_graph_destroy(fsx_graph g);
  * fsx_cnn_probe16: the same with A (32x16) and B (16x32) through the
  *   K = 16 instruction the convolutions use.
  * fsx_cnn_conv_fwd: y = bias + periodic 5x5 correlation of x, co_out <= 32
- *   outputs; H multiple of 4, W multiple of 32, W <= 128 for C = 32.
+ *   outputs; H multiple of 4, W multiple of 32, W <= 160 for C = 32
+ *   (LDS of the 4-row tile; wider images return FSX_ERR_ARG).
  * fsx_cnn_conv_fwd_act: a hidden layer (32 outputs) with the activation in
  *   the epilogue: a = BF16(GELU(y)) (32 channels) and, when y is not NULL,
  *   the FP32 pre-activations y the backward pass needs.
