@@ -39,6 +39,15 @@ budgets / spatial means     rate of change of the spectra under the oracle's
                             own step (order 1), injection = forcing rate
 Euler / RK2 / phase shifts  exact IF decay, order of convergence, alias-free
                             reduction of the shifted tendency
+forcing (tcrandom, aniso.)  injection = forcing rate (closed form), ring
+                            count, root choice vs numpy.roots
+Smagorinsky                 energy identity, traceless strain, shear closed
+                            form, brute-force DFT on 4^3
+cnn (closure network)       FFT correlation form of the periodic convolution,
+                            finite-difference gradients of every weight, bias
+                            and the input, unflipped-kernel mistake fails;
+                            rotation / mirror symmetry of the ns2d tendency
+                            (the training augmentation)
 ==========================  ===================================================
 """
 
