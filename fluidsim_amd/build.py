@@ -1,7 +1,8 @@
 """In-tree build of the HIP engine library for gfx950.
 
 ``build()`` always recompiles ``fluidsim_amd/libfluidsim_amd.so`` with hipcc
-(cross-compiling works without a GPU).
+(cross-compiling works without a GPU).  The engine is split over several
+translation units that compile in parallel.
 """
 
 import os
@@ -14,7 +15,14 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "libfluidsim_amd.so")
 
-SOURCES = ["fsx_engine.hip"]
+SOURCES = [
+    "fsx_engine.hip",
+    "fsx_api_ns2d.hip",
+    "fsx_api_ns3d.hip",
+    "fsx_api_strat.hip",
+    "fsx_api_diag.hip",
+    "fsx_api_cnn.hip",
+]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -34,18 +42,37 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    objs = []
+def _jobs():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(8, n, int(os.environ.get("MAX_JOBS", "8"))))
+
+
+def build(verbose=False, extra_flags=()):
+    objs, procs, pending = [], [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, capture_output=True, text=True)
-        if res.returncode != 0:
-            sys.stderr.write(res.stdout + res.stderr)
-            raise RuntimeError(f"hipcc failed on {src}")
+        pending.append((src, cmd))
         objs.append(obj)
+    failed = []
+    jobs = _jobs()
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            src, cmd = pending.pop(0)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                 stderr=subprocess.STDOUT, text=True)))
+        src, proc = procs.pop(0)
+        out, _ = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(src)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)

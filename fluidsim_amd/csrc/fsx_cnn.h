@@ -59,7 +59,7 @@ __device__ __forceinline__ f32x16 mfma_bf16(bf16x4 a, bf16x4 b, f32x16 c) {
 
 // D (32 x 32, row-major FP32) = A (32 x 8 BF16, row-major) x B (8 x 32 BF16,
 // row-major): one wave; checks the operand layout above.
-__global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint16_t* B,
+static __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t* A, const uint16_t* B,
                                                    float* D) {
   const int lane = threadIdx.x, m = lane & 31, g = lane >> 5;
   bf16x4 a, b;
@@ -82,7 +82,7 @@ __device__ __forceinline__ f32x16 mfma16_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
 
 // D (32 x 32, row-major FP32) = A (32 x 16 BF16, row-major) x B (16 x 32
 // BF16, row-major): one wave; checks the operand layout of the K = 16 form.
-__global__ __launch_bounds__(64) void k_mfma16_probe(const uint16_t* A, const uint16_t* B,
+static __global__ __launch_bounds__(64) void k_mfma16_probe(const uint16_t* A, const uint16_t* B,
                                                      float* D) {
   const int lane = threadIdx.x, m = lane & 31, g = lane >> 5;
   uint16_t ha[8], hb[8];
@@ -119,7 +119,7 @@ __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
 // XF32 (CI = 8 only): x is one FP32 channel x[B][H][W], rounded to BF16 and
 // zero-padded to 8 channels while it is staged.
 template <int CI, int FUSE = 0, bool XF32 = false>
-__global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
+static __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
                                                   const uint16_t* __restrict__ w,
                                                   const float* __restrict__ bias,
                                                   float* __restrict__ y, int H, int W,
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x
 
 // wt[c][24 - t][o] = w[o][t][c] for o < co, c < ci; wt has 32 rows and
 // CO_PAD input channels (zero-padded).
-__global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad, uint16_t* wt) {
+static __global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad, uint16_t* wt) {
   const int n = 32 * kCnnTaps * co_pad;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int o = i % co_pad, t = (i / co_pad) % kCnnTaps, c = i / (co_pad * kCnnTaps);
@@ -265,7 +265,7 @@ __host__ __device__ inline size_t conv_wgrad_lds(int ci, int W) {
 // its partial part[g][32][25 * CI]; k_cnn_reduce sums the partials.
 // Each wave owns the N tiles (32 columns of (tap, ci)) wid, wid + 4, ...
 template <int CI, int CO_IN>
-__global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x,
+static __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x,
                                                     const uint16_t* __restrict__ dy,
                                                     float* __restrict__ part, int B, int H,
                                                     int W) {
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__
 // interleaved partial sums (8 loads in flight per lane), which are combined
 // in a fixed order, then the 4 waves in a fixed order: deterministic, the
 // same result for every launch. Launch with (n + 63) / 64 workgroups of 256.
-__global__ __launch_bounds__(256) void k_cnn_reduce(const float* part, int ngroups, int n,
+static __global__ __launch_bounds__(256) void k_cnn_reduce(const float* part, int ngroups, int n,
                                                     float* out) {
   __shared__ float red[4][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_cnn_reduce(const float* part, int ngrou
 }
 
 // bias gradient partials: part[blk][o] = sum over the block's pixels of dy[p][o]
-__global__ __launch_bounds__(256) void k_cnn_bias_grad(const uint16_t* dy, long long npix,
+static __global__ __launch_bounds__(256) void k_cnn_bias_grad(const uint16_t* dy, long long npix,
                                                        int co, float* part) {
   __shared__ float red[256];
   const int o = threadIdx.x % 32, lane8 = threadIdx.x / 32;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void k_cnn_bias_grad(const uint16_t* dy, long 
 
 // a[p][c] = BF16(act(y[p][c])) for c < cin, 0 up to cpad; act = GELU
 // (gelu != 0) or identity
-__global__ void k_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu,
+static __global__ void k_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu,
                           uint16_t* a) {
   const long long n = npix * cpad;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -405,7 +405,7 @@ __global__ void k_cnn_act(const float* y, long long npix, int cin, int cpad, int
 
 // d[p][c] = BF16(g[p][c] * gelu'(y[p][c])) (g: gradient w.r.t. the GELU
 // output, channel stride gs; y: pre-activation, c channels)
-__global__ void k_cnn_act_bwd(const float* g, int gs, const float* y, long long npix, int c,
+static __global__ void k_cnn_act_bwd(const float* g, int gs, const float* y, long long npix, int c,
                               uint16_t* d) {
   const long long n = npix * c;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -418,7 +418,7 @@ __global__ void k_cnn_act_bwd(const float* g, int gs, const float* y, long long 
 
 // loss partials and gradient of L = mean((out - target)^2):
 // d[p][c] = BF16(2 (out - target) / n) for c = 0, 0 up to cpad
-__global__ __launch_bounds__(256) void k_cnn_mse(const float* out, const float* target,
+static __global__ __launch_bounds__(256) void k_cnn_mse(const float* out, const float* target,
                                                  long long npix, int cpad, uint16_t* d,
                                                  double* part) {
   __shared__ double red[256];
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void k_cnn_mse(const float* out, const float* 
 }
 
 // Adam (FP32 master weights) and the BF16 copy used by the kernels
-__global__ void k_cnn_adam(float* w, const float* gr, float* m1, float* m2, int n, float lr,
+static __global__ void k_cnn_adam(float* w, const float* gr, float* m1, float* m2, int n, float lr,
                            float b1, float b2, float eps, float c1, float c2, uint16_t* wb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float gi = gr[i];
@@ -456,7 +456,7 @@ __global__ void k_cnn_adam(float* w, const float* gr, float* m1, float* m2, int 
 }
 
 // BF16 copy of FP32 values (weights)
-__global__ void k_cnn_to_bf16(const float* in, int n, uint16_t* out) {
+static __global__ void k_cnn_to_bf16(const float* in, int n, uint16_t* out) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     out[i] = to_bf16(in[i]);
 }

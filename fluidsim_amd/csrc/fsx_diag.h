@@ -333,7 +333,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
+static __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
                                               double* partial) {
 #pragma clang fp contract(off)
   constexpr int Q = diag_nq(KIND);
@@ -451,7 +451,7 @@ struct DiagOut {
 // One wave per output value: the lanes sum strided subsets of the
 // contributing block records, then a fixed shuffle tree adds the lanes
 // (deterministic for a given grid).
-__global__ __launch_bounds__(256) void k_diag_reduce(const double* partial, DiagGeom g,
+static __global__ __launch_bounds__(256) void k_diag_reduce(const double* partial, DiagGeom g,
                                                      DiagOut o, double* out) {
   const int nq = g.nq;
   const long long n_kx = (g.flags & DIAG_KX) ? (long long)nq * g.nkxs : 0;
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(256) void k_diag_reduce(const double* partial, Diag
 // spectral fields ux, uy, d_x rot, d_y rot, d_x ux, d_y ux, d_x uy, d_y uy
 // (gradfft_from_fft: i kx a, i ky a) over all modes.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ns2d_budget_grads(const cpx<T>* rot, cpx<T>* w,
+static __global__ __launch_bounds__(256) void k_ns2d_budget_grads(const cpx<T>* rot, cpx<T>* w,
                                                            long long ke, int nkx, int ny,
                                                            double dkx, double dky) {
 #pragma clang fp contract(off)
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void k_ns2d_budget_grads(const cpx<T>* rot, cp
 // Frot = -ux d_x rot - uy (d_y rot + beta), Fx = -ux d_x ux - uy d_y ux,
 // Fy = -ux d_x uy - uy d_y uy, written over fields 0..2 of x.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ns2d_budget_prod(T* x, long long xe, double beta) {
+static __global__ __launch_bounds__(256) void k_ns2d_budget_prod(T* x, long long xe, double beta) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < xe;
        i += (long long)gridDim.x * blockDim.x) {
     const T ux = x[i], uy = x[xe + i];

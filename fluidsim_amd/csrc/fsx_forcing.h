@@ -111,7 +111,7 @@ __device__ inline cpx<double> norm_root(double a, double b, double c, int which)
 
 // One workgroup of 256 threads (the coarse box has a few hundred modes).
 template <typename T>
-__global__ __launch_bounds__(256) void k_forcing_norm(ForcingArgs<T> f) {
+static __global__ __launch_bounds__(256) void k_forcing_norm(ForcingArgs<T> f) {
   __shared__ double red_a[256], red_b[256];
   __shared__ double s_R[2];
   double pa = 0.0, pb = 0.0;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_forcing_norm(ForcingArgs<T> f) {
 
 // out[v][full(c)] += scale * Fc[v][c] over the whole coarse box.
 template <typename T>
-__global__ __launch_bounds__(256) void k_forcing_add(cpx<T>* out, long long ke,
+static __global__ __launch_bounds__(256) void k_forcing_add(cpx<T>* out, long long ke,
                                                      const cpx<T>* fc,
                                                      long long fvs, int nvar,
                                                      int nyc, int nzc, int nkxc,

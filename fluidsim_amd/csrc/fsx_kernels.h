@@ -195,7 +195,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 #endif
 
 template <typename T, int DIR, int EPT, int kT, class Op>
-__global__ __launch_bounds__(kT) FSX_STRIDED_ATTR void k_strided(Op op) {
+static __global__ __launch_bounds__(kT) FSX_STRIDED_ATTR void k_strided(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const StridedDesc<T>& d = op.d;
   cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -759,7 +759,7 @@ __device__ __forceinline__ int rpad(int lr, int x, int ldr) {
 #endif
 // At least FSX_XPASS_WAVES waves per SIMD (caps the VGPR count).
 template <typename T, int EPT, int kT, class Op>
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XPASS_WAVES)))
+static __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XPASS_WAVES)))
 void k_xpass(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const XDesc<T>& d = op.d;

@@ -221,7 +221,7 @@ __device__ __forceinline__ void rk_point(const RKArgs<T>& a, const Visc& visc, d
 // One Runge-Kutta stage over the kept modes from a tendency array.
 // One wavefront per kept (kz, ky) row, lanes along kx.
 template <typename T, int NV>
-__global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
+static __global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
   const Box& bx = a.box;
   const int nyk = bx.ym.nk();
   const int nrow = (bx.dim == 3 ? bx.zm.nk() : 1) * nyk;
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_rk_stage(RKArgs<T> a) {
 
 // Standalone projection of 3 fields over the full K array.
 template <typename T>
-__global__ __launch_bounds__(256) void k_project(cpx<T>* a, cpx<T>* b,
+static __global__ __launch_bounds__(256) void k_project(cpx<T>* a, cpx<T>* b,
                                                  cpx<T>* c, int kind, Box box) {
   const size_t n = (size_t)box.nz * box.ny * box.nkx;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void k_project(cpx<T>* a, cpx<T>* b,
 
 // Zero the dealiased modes of nf fields (mask or index box).
 template <typename T>
-__global__ __launch_bounds__(256) void k_dealias(cpx<T>* base, size_t fstride,
+static __global__ __launch_bounds__(256) void k_dealias(cpx<T>* base, size_t fstride,
                                                  int nf, Box box,
                                                  const unsigned char* mask) {
   const size_t n = (size_t)box.nz * box.ny * box.nkx;
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_dealias(cpx<T>* base, size_t fstride,
 //   kind 1 (3D, fields = v):    E, epsK, epsK_hypo
 //   kind 2 (strat, v and b):    E_K, E_A (= |b|^2 / (2 N^2)), epsK, epsA
 template <typename T>
-__global__ __launch_bounds__(256) void k_sums(const cpx<T>* f0,
+static __global__ __launch_bounds__(256) void k_sums(const cpx<T>* f0,
                                               const cpx<T>* f1,
                                               const cpx<T>* f2,
                                               const cpx<T>* f3, int kind,
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_sums(const cpx<T>* f0,
 // CFL update (fluidsim/base/time_stepping/base.py:320-366): tmp =
 // sum_i max|u_i| / dx_i, dt_CFL = CFL / tmp, new = min(dt_CFL, dt_max),
 // kept unless the relative change exceeds 2 %.
-__global__ void k_cfl_update(const unsigned long long* maxima, int ncomp,
+static __global__ void k_cfl_update(const unsigned long long* maxima, int ncomp,
                              double dx, double dy, double dz, double cfl,
                              double dt_max, double* dt) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -380,7 +380,7 @@ __global__ void k_cfl_update(const unsigned long long* maxima, int ncomp,
   if (fabs(*dt - nw) / nw > 0.02) *dt = nw;
 }
 
-__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+static __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -408,7 +408,7 @@ struct IFCombine {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_if_combine(IFCombine<T> c) {
+static __global__ __launch_bounds__(256) void k_if_combine(IFCombine<T> c) {
   const Box& bx = c.box;
   const double dt = *c.dt;
   const long long n = c.ke;
@@ -464,7 +464,7 @@ __device__ __forceinline__ int resize_src(int o, int n_out, int n_in) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_resize(const cpx<T>* in, int nz_in, int ny_in,
+static __global__ __launch_bounds__(256) void k_resize(const cpx<T>* in, int nz_in, int ny_in,
                                                 int nkx_in, cpx<T>* out, int nz_out,
                                                 int ny_out, int nkx_out, int nf) {
   const long long ko = (long long)nz_out * ny_out * nkx_out;

@@ -333,7 +333,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off
 // Global accesses go through one buffer resource per (wave, field) that
 // spans the wave's valid rows.
 template <typename T, int LOGM, int LOGP, int kT, class Op>
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XR_WAVES)))
+static __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XR_WAVES)))
 void k_xrow(XRDesc<T> d, Op op) {
   using PL = RowPlan<LOGM, LOGP>;
   constexpr int P = PL::P, G = PL::G, M = PL::M;

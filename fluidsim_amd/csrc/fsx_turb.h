@@ -37,7 +37,7 @@ __device__ __forceinline__ cpx<T> i_times(cpx<T> a, double s) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_strain_rate(const cpx<T>* v, cpx<T>* S,
+static __global__ __launch_bounds__(256) void k_strain_rate(const cpx<T>* v, cpx<T>* S,
                                                      long long ke, KGrid g) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < ke;
        i += (long long)gridDim.x * blockDim.x) {
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_strain_rate(const cpx<T>* v, cpx<T>* S,
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_smag_phys(T* S, long long xe, T coef) {
+static __global__ __launch_bounds__(256) void k_smag_phys(T* S, long long xe, T coef) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < xe;
        i += (long long)gridDim.x * blockDim.x) {
     const T sxx = S[i], syy = S[xe + i], szz = S[2 * xe + i];
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_smag_phys(T* S, long long xe, T coef) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_smag_div(const cpx<T>* gk, cpx<T>* out,
+static __global__ __launch_bounds__(256) void k_smag_div(const cpx<T>* gk, cpx<T>* out,
                                                   long long ke, KGrid g,
                                                   int accumulate) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < ke;
