@@ -1,12 +1,14 @@
 #!/usr/bin/env python
 """Benchmark: RK4 steps/s of the ns3d solver on one MI355X.
 
-Workload (default): ns3d, periodic (2 pi)^3, 512^3, FP32, RK4, order-8
+Workload (default): ns3d, periodic (2 pi)^3, 1024^3, FP32, RK4, order-8
 hyperviscosity with k_max * eta = 1, seeded isotropic solenoidal initial
-velocity with E(k) ~ k^4 exp(-2 (k/4)^2) normalised to unit energy, fixed dt
-taken from the initial CFL condition at 0.5.  A step is one full RK4 step
-(4 fused tendency evaluations + 4 Runge-Kutta stages) with the state
-resident in HBM.
+velocity with E(k) ~ k^4 exp(-2 (k/4)^2) normalised to unit energy (drawn on
+the device at 1024^3), fixed dt taken from the initial CFL condition at 0.5.
+A step is one full RK4 step (4 fused tendency evaluations + 4 Runge-Kutta
+stages) with the state resident in HBM.  The same line carries the 512^3
+FP32 and FP64 rates of the workload (``cfg2``), the FP32-vs-FP64 and
+FP64-oracle accuracy (``accuracy``) and the CPU baseline.
 
 ``--ensemble``: the batched ns2d ensemble instead (512 independent FP32
 runs: 256 at 128^2, 192 at 256^2, 64 at 512^2, seed = run index, nu_2
@@ -55,9 +57,12 @@ def b_alg_ns3d(n, prec):
     return 36 * (bX + bK) + 51 * bK
 
 
-def make_sim(n, precision, seed=0, vv=None):
+def make_sim(n, precision, seed=0, vv=None, device_init=False):
+    """ns3d n^3 sim of the workload; the initial field comes from vv, from the
+    host generator, or (device_init, for 1024^3) from the device generator."""
     from fluidsim_amd.solvers.ns3d.solver import Simul
-    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+    from fluidsim_amd.synthetic import (ns3d_spectrum_vv_device, ns3d_spectrum_vv_fft,
+                                        nu8_for_kmax_eta)
 
     params = Simul.create_default_params()
     params.oper.nx = params.oper.ny = params.oper.nz = n
@@ -71,12 +76,17 @@ def make_sim(n, precision, seed=0, vv=None):
     params.output.periods_print.print_stdout = 0
     sim = Simul(params)
     cdt = np.complex64 if precision == "float32" else np.complex128
-    if vv is None:
-        vv = ns3d_spectrum_vv_fft(n, seed=seed)
-    # the generator rounds its FP64 field at the end: a cast of the FP64
-    # field is the FP32 field of the same seed
-    sim.state.init_statespect_from(vx_fft=vv[0].astype(cdt), vy_fft=vv[1].astype(cdt),
-                                   vz_fft=vv[2].astype(cdt))
+    if vv is None and device_init:
+        vv = ns3d_spectrum_vv_device(sim.oper, seed=seed)
+        sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+        del vv
+    else:
+        if vv is None:
+            vv = ns3d_spectrum_vv_fft(n, seed=seed)
+        # the generator rounds its FP64 field at the end: a cast of the FP64
+        # field is the FP32 field of the same seed
+        sim.state.init_statespect_from(vx_fft=vv[0].astype(cdt), vy_fft=vv[1].astype(cdt),
+                                       vz_fft=vv[2].astype(cdt))
     # fixed dt from the initial CFL condition
     sim.time_stepping.compute_time_increment_CLF()
     params.time_stepping.USE_CFL = False
@@ -163,15 +173,10 @@ def _device_rel_l2(torch, a, ref):
     return (num / den) ** 0.5
 
 
-def ns3d_accuracy(torch, n_bench, steps_bench, vv=None, n_oracle=128, steps_oracle=5,
-                  seed=0):
-    """The FP32 engine against FP64: (1) at n_oracle, FP64 and FP32 engine
-    states after steps_oracle fixed-dt steps against the FP64 oracle on the
-    same seeded fields; (2) at the bench size, the FP32 engine against the
-    FP64 engine after steps_bench steps at the same dt, compared on the
-    device (state, and the shell energy spectrum of the engine's FP64
-    diagnostics pass); the FP64 engine holds the oracle to 1e-11 at every
-    size the tests run."""
+def ns3d_accuracy(torch, n_oracle=128, steps_oracle=5, seed=0):
+    """The FP64 and FP32 engines against the FP64 oracle at n_oracle: states
+    after steps_oracle fixed-dt steps on the same seeded fields (relative L2)
+    and their shell energy spectra."""
     from oracle import operators as O
     from oracle.solvers import NS3D
     from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
@@ -202,27 +207,63 @@ def ns3d_accuracy(torch, n_bench, steps_bench, vv=None, n_oracle=128, steps_orac
     res["oracle_case"] = (f"ns3d {n_oracle}^3, {steps_oracle} RK4 steps at the FP64 "
                           f"initial-CFL dt, same seeded fields, FP64 numpy oracle")
     del s64
-    # bench size (at most 512^3, so that both engines fit in HBM together):
-    # FP32 against the FP64 engine, on the device
-    if n_bench > 512:
-        n_bench, vv = 512, None
-    if vv is None:
-        vv = ns3d_spectrum_vv_fft(n_bench, seed=seed)
-    s64 = make_sim(n_bench, "float64", seed, vv=vv)
-    s32 = make_sim(n_bench, "float32", seed, vv=vv)
+    gc.collect()
+    return res
+
+
+def _timed_steps(torch, ts, nsteps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        ts.one_time_step_computation()
+        ts.t += ts.deltat
+        ts.it += 1
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cfg2_legs(torch, n=512, warmup=2, steps=10, steps_total=100, seed=0):
+    """The 512^3 workload in FP32 and FP64 on the same seeded fields and the
+    same dt: RK4 steps/s of each (warmup, then `steps` timed steps with the
+    state in HBM) and its step roofline; then both continue to steps_total
+    steps and the FP32 state and shell energy spectrum are compared with the
+    FP64 engine's on the device (the FP64 engine holds the oracle to 1e-11
+    at every size the tests run)."""
+    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft
+
+    vv = ns3d_spectrum_vv_fft(n, seed=seed)
+    with contextlib.redirect_stdout(sys.stderr):
+        s64 = make_sim(n, "float64", seed, vv=vv)
+        s32 = make_sim(n, "float32", seed, vv=vv)
+    del vv
     s32.time_stepping.deltat = s64.time_stepping.deltat
-    for sim in (s64, s32):
-        for _ in range(steps_bench):
+    res = {"case": (f"ns3d {n}^3 RK4, same workload and seeded fields as the headline "
+                    f"(host generator), same fixed dt in both precisions")}
+    for prec, sim in (("float32", s32), ("float64", s64)):
+        ts = sim.time_stepping
+        for _ in range(warmup):
+            ts.one_time_step_computation()
+        el = _timed_steps(torch, ts, steps)
+        sps = steps / el
+        balg = b_alg_ns3d(n, 4 if prec == "float32" else 8)
+        res[prec] = {"steps_per_s": sps, "ms_per_step": el / steps * 1e3,
+                     "steps": steps, "warmup": warmup,
+                     "step_roofline": {"B_alg_GB_per_step": balg / 1e9,
+                                       "achieved_GBps": balg * sps / 1e9,
+                                       "frac": balg * sps / 1e9 / PEAK_HBM_GBPS}}
+    for sim in (s32, s64):
+        for _ in range(steps_total - warmup - steps):
             sim.time_stepping.one_time_step_computation()
     torch.cuda.synchronize()
     E = [sim.output.spectra.compute(kzkh=False)[1]["spectra_E"] for sim in (s32, s64)]
     res["float32_vs_float64_engine"] = {
-        "case": f"ns3d {n_bench}^3, {steps_bench} RK4 steps at the same dt",
+        "case": f"ns3d {n}^3, {steps_total} RK4 steps at the same dt",
         "state_rel_l2": _device_rel_l2(torch, s32.state.state_spect.data,
                                        s64.state.state_spect.data),
-        "energy_spectrum": _spectrum_error(E[0], E[1], n_bench // 3)}
+        "energy_spectrum": _spectrum_error(E[0], E[1], n // 3)}
     del s32, s64
     gc.collect()
+    torch.cuda.empty_cache()
     return res
 
 
@@ -446,37 +487,79 @@ def bench_strat(args, torch, _lib, copy_gbps):
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(n_sample, n_metric, steps=1):
-    """FP64 oracle (numpy + scipy.fft on all host cores of this process)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_step_time(n, workers, steps=2):
+    """Median wall time of `steps` RK4 steps of the ns3d workload at n^3 with
+    the FP64 oracle (scipy.fft workers, numpy elementwise), after one
+    warm-up step (fluidsim-bench style: fixed dt, no output)."""
     from oracle import operators as O
     from oracle.solvers import NS3D
     from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
 
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    O.set_workers(cores)
-    op = O.Oper3D(n_sample, n_sample, n_sample)
-    sim = NS3D(op, nu_8=nu8_for_kmax_eta(n_sample), USE_CFL=True, cfl_coef=0.5)
-    vv = ns3d_spectrum_vv_fft(n_sample, seed=0)
+    O.set_workers(workers)
+    op = O.Oper3D(n, n, n)
+    sim = NS3D(op, nu_8=nu8_for_kmax_eta(n), USE_CFL=True, cfl_coef=0.5)
+    vv = ns3d_spectrum_vv_fft(n, seed=0)
     sim.state_spect[:] = np.array(vv)
     del vv
     sim.compute_time_increment_CLF()
-    t0 = time.perf_counter()
+    sim.USE_CFL = False
+    sim.one_time_step_computation()
+    ts = []
     for _ in range(steps):
+        t0 = time.perf_counter()
         sim.one_time_step_computation()
-    dt = (time.perf_counter() - t0) / steps
-    N1 = n_sample**3
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(n_metric, n_all=256, n_one=128):
+    """The FP64 oracle on the host cores: all cores of this process (scipy.fft
+    workers = cores, the analogue of mpirun -np cores) at n_all^3 and one
+    core (fluidsim's default sequential mode) at n_one^3, each scaled by
+    N log N to n_metric^3."""
+    import scipy
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
     N2 = n_metric**3
-    scale = (N2 * math.log2(N2)) / (N1 * math.log2(N1))
+
+    def scaled(n, workers):
+        med, ts = _oracle_step_time(n, workers)
+        N1 = n**3
+        scale = (N2 * math.log2(N2)) / (N1 * math.log2(N1))
+        return med, ts, scale
+
+    med, ts, scale = scaled(n_all, cores)
+    med1, ts1, scale1 = scaled(n_one, 1)
     return {
-        "value": 1.0 / (dt * scale),
+        "value": 1.0 / (med * scale),
         "unit": "RK4 steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": (
-            f"{steps} RK4 step of the same ns3d workload at {n_sample}^3 FP64 "
-            f"({dt:.2f} s/step with scipy.fft workers={cores}, numpy elementwise "
-            f"single-threaded), scaled by N log N (x{scale:.2f}) to {n_metric}^3"
-        ),
+        "sample": (f"median of {len(ts)} RK4 steps (after 1 warm-up) of the same ns3d "
+                   f"workload at {n_all}^3 FP64 with the oracle: {med:.2f} s/step "
+                   f"(scipy.fft workers={cores}, numpy elementwise single-threaded), "
+                   f"scaled by N log N (x{scale:.1f}) to {n_metric}^3"),
+        "cpu_model": _cpu_model(),
+        "numpy": np.__version__,
+        "scipy": scipy.__version__,
+        "single_core": {
+            "value": 1.0 / (med1 * scale1),
+            "cores": 1,
+            "sample": (f"median of {len(ts1)} RK4 steps (after 1 warm-up) at {n_one}^3 "
+                       f"FP64, one core: {med1:.2f} s/step, scaled by N log N "
+                       f"(x{scale1:.0f}) to {n_metric}^3"),
+        },
     }
 
 
@@ -934,15 +1017,19 @@ def read_traffic(kernel_key):
         return None
 
 
+T_START = time.perf_counter()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--precision", default="float32", choices=["float32", "float64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-n", type=int, default=256,
+                    help="grid of the all-cores CPU baseline sample (one core: half)")
     ap.add_argument("--ensemble", action="store_true",
                     help="time the batched ns2d ensemble (512 runs) instead")
     ap.add_argument("--closure", action="store_true",
@@ -965,6 +1052,10 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-vs-FP64 accuracy runs of the ns3d bench")
     ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--cfg2-n", type=int, default=512,
+                    help="grid of the FP32 / FP64 legs of the ns3d workload")
+    ap.add_argument("--no-cfg2", action="store_true",
+                    help="skip the FP32 / FP64 legs at --cfg2-n")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first "
                          "(calibration kernel for the HBM byte counters)")
@@ -1002,11 +1093,11 @@ def main():
         del src, dst
     print(f"[bench] building ns3d {n}^3 {args.precision}", file=sys.stderr, flush=True)
     # the solver's own progress messages go to stderr: stdout is the JSON line
-    from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft
-
-    vv0 = ns3d_spectrum_vv_fft(n, seed=0)
+    t_build = time.perf_counter()
     with contextlib.redirect_stdout(sys.stderr):
-        sim = make_sim(n, args.precision, vv=vv0)
+        sim = make_sim(n, args.precision, device_init=n > 512)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
     ts = sim.time_stepping
     plan = sim.oper.plan.handle
     print(f"[bench] dt = {ts.deltat:.6g}; warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -1064,11 +1155,13 @@ def main():
         "config": {
             "workload": (
                 f"ns3d {n}^3 {args.precision} RK4, nu_8 with k_max*eta=1, "
-                "E(k)~k^4 exp(-2(k/4)^2) unit-energy solenoidal init (seed 0), "
-                "fixed dt from initial CFL 0.5, 2/3 dealiasing"
+                "E(k)~k^4 exp(-2(k/4)^2) unit-energy solenoidal init (seed 0, "
+                + ("device generator" if n > 512 else "host generator")
+                + "), fixed dt from initial CFL 0.5, 2/3 dealiasing"
             ),
             "grid": [n, n, n],
             "parallelism": "single GPU",
+            "setup_s": t_build,
         },
         "roofline": {
             "bound": "hbm",
@@ -1078,6 +1171,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBPS,
             "traffic": traffic,
+            "traffic_source": ("HBM bytes per launch of this kernel at this workload from "
+                               "rocprofv3 FETCH_SIZE / WRITE_SIZE runs of bench.py, "
+                               "committed in profiles/traffic.json (not measured in "
+                               "this run)"),
             "copy_peak_measured": copy_gbps,
             "frac_of_copy": achieved / copy_gbps,
         },
@@ -1093,16 +1190,23 @@ def main():
     del sim, ts
     gc.collect()
     torch.cuda.empty_cache()
+    if not args.no_cfg2:
+        m = args.cfg2_n
+        print(f"[bench] {m}^3 FP32 and FP64 legs", file=sys.stderr, flush=True)
+        result["cfg2"] = cfg2_legs(torch, m, steps_total=args.accuracy_steps)
     if not args.no_accuracy:
         print("[bench] accuracy: FP32 and FP64 engine vs the FP64 oracle", file=sys.stderr,
               flush=True)
         with contextlib.redirect_stdout(sys.stderr):
-            result["accuracy"] = ns3d_accuracy(torch, n, args.accuracy_steps, vv=vv0)
-        del vv0
+            result["accuracy"] = ns3d_accuracy(torch)
+        if "cfg2" in result:
+            result["accuracy"]["float32_vs_float64_engine"] = result["cfg2"].pop(
+                "float32_vs_float64_engine")
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(args.cpu_n, n)
+        result["cpu_baseline"] = cpu_baseline(n, args.cpu_n, args.cpu_n // 2)
+    result["bench_wall_s"] = time.perf_counter() - T_START
     print(json.dumps(result), flush=True)
 
 

@@ -84,11 +84,15 @@ class SpecificOutput:
         self._save(self.compute())
         self.t_last_save = self.sim.time_stepping.t
 
+    def due(self, tsim):
+        return bool(self.period_save) and (
+            self.t_last_save is None or tsim - self.t_last_save >= self.period_save)
+
     def _online_save(self):
         if not self.period_save:
             return
         tsim = self.sim.time_stepping.t
-        if self.t_last_save is None or tsim - self.t_last_save >= self.period_save:
+        if self.due(tsim):
             self.t_last_save = tsim
             self._save(self.compute())
 

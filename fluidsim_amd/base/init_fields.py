@@ -1,20 +1,22 @@
-"""Initial fields (fluidsim/base/init_fields.py:26-127, 435-443).
+"""Initial fields (fluidsim/base/init_fields.py:26-127, 140-297, 435-443).
 
 Types: "in_script" (the state stays zero until the script calls
 ``sim.state.init_statespect_from(...)``), "constant", "from_simul"
 (``sim.init_fields.get_state_from_simul(sim_in)`` copies the state of
 another simulation, changing the resolution on the device;
-fluidsim/base/init_fields.py:340-432), and the solver-specific ones
-registered by subclasses (e.g. "noise").  "from_file" (restart from a
-state file) is not available: no HDF5 / netCDF library in this build.
+fluidsim/base/init_fields.py:340-432), "from_file" (restart from a state
+file of fluidsim_amd.base.phys_fields, ``params.init_fields.from_file.path``)
+and the solver-specific ones registered by subclasses (e.g. "noise").
 """
+
+import numpy as np
 
 from .. import _lib
 from ..params import Parameters
 
 
 class InitFieldsBase:
-    available_types = ("in_script", "constant", "from_simul")
+    available_types = ("in_script", "constant", "from_simul", "from_file")
 
     @classmethod
     def _complete_params_with_default(cls, params, info_solver=None):
@@ -28,6 +30,7 @@ class InitFieldsBase:
             },
         )
         params.init_fields._set_child("constant", attribs={"value": 1.0})
+        params.init_fields._set_child("from_file", attribs={"path": ""})
         cls._complete_params_specific(params)
 
     @classmethod
@@ -53,6 +56,46 @@ class InitFieldsBase:
                 f"(available: {self.available_types})"
             )
         meth()
+
+    def init_from_file(self):
+        """InitFieldsFromFile.__call__ (fluidsim/base/init_fields.py:157-297):
+        the file's grid and box must be the simulation's; the physical
+        fields the state needs are read (missing ones are zero), the
+        spectral state is their transform, and time and iteration are the
+        file's."""
+        from .phys_fields import StateFile
+
+        sim = self.sim
+        path = str(self.params.init_fields.from_file.path)
+        f = StateFile(path)
+        print("Load state from file:\n[...]" + path[-75:])
+        try:
+            oper_file = f.params.get("oper", {})
+            po = self.params.oper
+            for r in f.attrs.get("axes", []):
+                nr, Lr = f"n{r}", f"L{r}"
+                if po[nr] != oper_file.get(nr):
+                    raise ValueError(
+                        "this is not a correct state for this simulation\n"
+                        "self.{0} != params_file.{0}".format(nr))
+                if Lr in oper_file and po[Lr] != oper_file[Lr]:
+                    raise ValueError(
+                        "this is not a correct state for this simulation\n"
+                        "self.params.oper.{0} != params_file.{0}".format(Lr))
+            state = sim.state
+            state._state_phys.initialize(0.0)
+            needed = sim.info_solver.classes.State.keys_phys_needed
+            for k in needed:
+                if k in f.keys:
+                    state._state_phys.set_var(k, np.ascontiguousarray(f[k]))
+            t = float(f.state_attrs.get("time", 0.0))
+            it = int(f.state_attrs.get("it", 0))
+        finally:
+            f.close()
+        state.statespect_from_statephys()
+        state.invalidate_phys()
+        sim.time_stepping.t = t
+        sim.time_stepping.it = it
 
     def init_from_simul(self):
         """InitFieldsFromSimul.__call__ (init_fields.py:343-344)."""

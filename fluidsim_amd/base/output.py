@@ -5,8 +5,9 @@ specific outputs of fluidsim_amd.base.diagnostics, computed on the device:
 spatial_means, spectra and spect_energy_budg (periods_save.*), created at
 post_init, first saved when the state is initialised, then every period.
 Files (stdout.txt, params.json, spatial_means.txt, spectra / budget .npz) go
-to path_run when params.output.HAS_TO_SAVE.  Physical-field state files
-and plots are not implemented.
+to path_run when params.output.HAS_TO_SAVE, and so do the physical-field
+state files of fluidsim_amd.base.phys_fields (periods_save.phys_fields, and
+one at the end of a run that saves).  Plots are not implemented.
 """
 
 import json
@@ -38,7 +39,8 @@ class OutputBasePseudoSpectral:
             params.output._set_child(tag, attribs={"HAS_TO_PLOT_SAVED": False})
         params.output._set_child("periods_print", attribs={"print_stdout": 1.0})
         params.output._set_child("periods_plot", attribs={"phys_fields": 0})
-        params.output._set_child("phys_fields", attribs={"field_to_plot": ""})
+        params.output._set_child("phys_fields",
+                                 attribs={"field_to_plot": "", "file_with_it": False})
 
     def __init__(self, sim):
         self.sim = sim
@@ -57,7 +59,29 @@ class OutputBasePseudoSpectral:
         self.name_solver = sim.info_solver.short_name
         self._has_to_save = bool(p.output.HAS_TO_SAVE)
         base = os.environ.get("FLUIDSIM_PATH", os.path.expanduser("~/Sim_data"))
-        self.path_run = os.path.join(base, p.output.sub_directory, name)
+        if not p.NEW_DIR_RESULTS and getattr(p, "path_run", ""):
+            # restart in the directory of the run (fluidsim/util/util.py:384-385)
+            self.path_run = str(p.path_run)
+            self.name_run = os.path.basename(os.path.normpath(self.path_run))
+        else:
+            self.path_run = os.path.join(base, p.output.sub_directory, name)
+            if self._has_to_save:
+                # a new directory per run: runs started within the same
+                # second get a numbered suffix
+                i = 0
+                while True:
+                    try:
+                        os.makedirs(self.path_run)
+                        break
+                    except FileExistsError:
+                        i += 1
+                        self.name_run = f"{name}_{i}"
+                        self.path_run = os.path.join(base, p.output.sub_directory,
+                                                     self.name_run)
+        if "path_run" in p:
+            p.path_run = self.path_run
+        else:
+            p._set_attrib("path_run", self.path_run)
         self._t_last = {}
         self._end_of_simul = False
         self._specific = []
@@ -69,12 +93,16 @@ class OutputBasePseudoSpectral:
     _cls_spect_energy_budg = None
 
     def post_init(self):
+        from .phys_fields import PhysFields
+
         for name in ("spatial_means", "spectra", "spect_energy_budg"):
             cls = getattr(self, "_cls_" + name)
             if cls is not None:
                 obj = cls(self)
                 setattr(self, name, obj)
                 self._specific.append(obj)
+        self.phys_fields = PhysFields(self)
+        self._specific.append(self.phys_fields)
         if self._has_to_save:
             os.makedirs(self.path_run, exist_ok=True)
             with open(os.path.join(self.path_run, "params.json"), "w") as f:
@@ -100,6 +128,19 @@ class OutputBasePseudoSpectral:
         self._do_outputs(force=True)
         for obj in self._specific:
             obj._init_files()
+
+    def outputs_due(self):
+        """Whether a print or a periodic save is due at the current time (such
+        outputs read dt and the forcing of the coming step)."""
+        t = self.sim.time_stepping.t
+        per = self.params.output.periods_print.print_stdout
+        last = self._t_last.get("print")
+        if per and (last is None or t - last >= per):
+            return True
+        for obj in self._specific:
+            if obj.due(t):
+                return True
+        return False
 
     def one_time_step(self):
         self._do_outputs(force=False)
@@ -132,7 +173,11 @@ class OutputBasePseudoSpectral:
         return self.spatial_means.load()
 
     def end_of_simul(self, total_time=None):
+        """fluidsim/base/output/base.py:354-361: a run that saves writes its
+        final state file (unless one was written at this time)."""
         self._end_of_simul = True
+        if self._has_to_save and self.phys_fields.t_last_save < self.sim.time_stepping.t:
+            self.phys_fields.save()
 
     # ---------------------------------------------------------------- sums
     def _sums(self, kind, fields, N2=0.0):

@@ -252,10 +252,39 @@ class TimeSteppingPseudoSpectral:
             # base.py:212-213 (the normalisation itself runs on the device
             # inside the first stage, after the CFL update)
             self.sim.forcing.compute()
+        if self.sim.output.outputs_due():
+            # outputs at t read dt and the forcing of this step: update both
+            # first, as base.py:208-244 orders them (the step then skips them)
+            self._update_dt_and_forcing()
         self.sim.output.one_time_step()
         self.one_time_step_computation()
         self.t += self.deltat
         self.it += 1
+
+    # set by _update_dt_and_forcing: the coming step's CFL update and forcing
+    # normalisation are done
+    _pre_done = False
+
+    def _update_dt_and_forcing(self):
+        """CFL update (base.py:320-366) and forcing normalisation of the coming
+        step on the current state, ahead of the outputs; the same device
+        work the first stage of the step would otherwise do, so the step is
+        unchanged."""
+        p = self.params.time_stepping
+        sim = self.sim
+        maker = (sim.forcing.forcing_maker
+                 if getattr(sim, "is_forcing_enabled", False) else None)
+        pending = maker is not None and getattr(maker, "_pending", False)
+        if not (p.USE_CFL or pending):
+            return
+        if p.USE_CFL:
+            self.compute_time_increment_CLF()
+        if pending:
+            self._push_dt()
+            _lib.call("fsx_forcing_normalize", sim.oper.plan.handle,
+                      _lib.dptr(sim.state.state_spect.data), self._stream())
+            maker._pending = False
+        self._pre_done = True
 
     def is_simul_completed(self):
         p = self.params.time_stepping
@@ -312,8 +341,9 @@ class TimeSteppingPseudoSpectral:
             getattr(sim, "is_turb_model_enabled", False)
             and getattr(sim, "_turb_model_in_tendency", True)
         )
+        use_cfl = p.USE_CFL and not self._pre_done
         if fused and hasattr(sim, "_engine_rk_stage"):
-            cfl = float(self.CFL) if p.USE_CFL else 0.0
+            cfl = float(self.CFL) if use_cfl else 0.0
             for stage in range(self._scheme):
                 sim._engine_rk_stage(self._scheme, stage, S, tmp, t1, self._visc,
                                      cfl if stage == 0 else 0.0,
@@ -325,10 +355,10 @@ class TimeSteppingPseudoSpectral:
         nvar = S.shape[0]
         proj = sim._engine_projection
         novz = int(getattr(sim, "no_vz_kz0", False))
-        if p.USE_CFL:
+        if use_cfl:
             _lib.call("fsx_reset_maxima", plan, s)
-        sim._engine_tendency(S, N, want_maxima=bool(p.USE_CFL))
-        if p.USE_CFL:
+        sim._engine_tendency(S, N, want_maxima=bool(use_cfl))
+        if use_cfl:
             _lib.call("fsx_cfl_update", plan, sim._cfl_ncomp, float(self.CFL),
                       float(self.deltat_max), s)
         if maker is not None and getattr(maker, "_pending", False):
@@ -373,7 +403,7 @@ class TimeSteppingPseudoSpectral:
         plan = sim.oper.plan.handle
         s = self._stream()
         self._push_dt()
-        if p.USE_CFL:
+        if p.USE_CFL and not self._pre_done:
             self.compute_time_increment_CLF()
         maker = (sim.forcing.forcing_maker
                  if getattr(sim, "is_forcing_enabled", False) else None)
@@ -441,10 +471,13 @@ class TimeSteppingPseudoSpectral:
     def one_time_step_computation(self):
         """pseudo_spect.py:236-243 / ns3d/time_stepping.py:8-20; projection
         and dealiasing of the new state are part of the last stage."""
-        if self._scheme == 0:
-            self._time_step_generic()
-        else:
-            self._time_step_RK()
+        try:
+            if self._scheme == 0:
+                self._time_step_generic()
+            else:
+                self._time_step_RK()
+        finally:
+            self._pre_done = False
         sim = self.sim
         sim.state.invalidate_phys()
         sc = self._read_scalars()

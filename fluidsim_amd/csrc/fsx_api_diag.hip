@@ -148,13 +148,21 @@ int diag_dispatch(int kind, const DiagIn& in, const Box& box, const DiagGeom& g,
   }
 }
 
-int grow(double*& buf, size_t& cap, size_t n) {
-  if (cap >= n) return 0;
-  if (buf) (void)hipFree(buf);
-  buf = nullptr;
-  cap = 0;
-  FSX_HIP(hipMalloc((void**)&buf, n * sizeof(double)));
-  cap = n;
+// Block records and output of fsx_diag for one kind and flag set: the
+// quantities of a launch (qstep) and the sizes in doubles.
+int diag_sizes(const fsx_plan_s* p, int kind, int flags, size_t& part, size_t& out,
+               int& qstep) {
+  DiagGeom g;
+  DiagOut o;
+  long long total;
+  int rc = diag_geom(p, kind, flags, g, o, total);
+  if (rc) return rc;
+  const int per_q = diag_per_quantity(g);
+  if (per_q > kDiagLdsDoubles) return fail(FSX_ERR_ARG, "grid too large for the LDS histograms");
+  qstep = std::max(1, std::min(o.Q, kDiagLdsDoubles / per_q));
+  const size_t nb = (size_t)g.nz * g.nchunk;
+  part = nb * (size_t)diag_record(g, 0, qstep);
+  out = (size_t)total;
   return 0;
 }
 
@@ -175,6 +183,32 @@ int ns2d_budget(fsx_plan_s* p, const void* rot, double beta, void* wk, void* wx,
 }
 
 }  // namespace
+
+// Buffers of fsx_diag, allocated with the plan: sized for the largest block
+// records and output over the diagnostics kinds of the plan's dimension with
+// every histogram (fewer histograms need less).  A grid too large for the
+// LDS histograms gets none, and fsx_diag refuses it.
+int fsx_diag_alloc(fsx_plan_s* p) {
+  size_t part = 0, out = 0;
+  for (int kind = 0; kind < DIAG_NKINDS; ++kind) {
+    const bool kind2d = kind == DIAG_NS2D_SPECTRA || kind == DIAG_NS2D_BUDGET ||
+                        kind == DIAG_NS2D_MEANS;
+    if (kind2d != (p->dim == 2)) continue;
+    const bool means = kind >= DIAG_NS2D_MEANS;
+    const int flags = means ? 0 : (p->dim == 3 ? 31 : 31 & ~(DIAG_KZ | DIAG_KZKH));
+    size_t pa = 0, ou = 0;
+    int qstep = 1;
+    if (diag_sizes(p, kind, flags, pa, ou, qstep)) return 0;
+    part = std::max(part, pa);
+    out = std::max(out, ou);
+  }
+  if (!part || !out) return 0;
+  FSX_HIP(hipMalloc((void**)&p->d_dpart, part * sizeof(double)));
+  p->dpart_cap = part;
+  FSX_HIP(hipMalloc((void**)&p->d_dout, out * sizeof(double)));
+  p->dout_cap = out;
+  return 0;
+}
 
 extern "C" {
 
@@ -230,16 +264,14 @@ int fsx_diag(fsx_plan p, int kind, int flags, const void* const* fields, int nfi
   in.visc.nu4 = params[2];
   in.visc.nu8 = params[3];
   in.visc.num4 = params[4];
-  const int per_q = diag_per_quantity(g);
-  if (per_q > kDiagLdsDoubles) return fail(FSX_ERR_ARG, "grid too large for the LDS histograms");
   const int Q = o.Q;
-  const int qstep = std::max(1, std::min(Q, kDiagLdsDoubles / per_q));
-  const size_t nb = (size_t)g.nz * g.nchunk;
-  DiagGeom gmax = g;
-  rc = grow(p->d_dpart, p->dpart_cap, nb * (size_t)diag_record(gmax, 0, qstep));
+  size_t need_part = 0, need_out = 0;
+  int qstep = 1;
+  rc = diag_sizes(p, kind, flags, need_part, need_out, qstep);
   if (rc) return rc;
-  rc = grow(p->d_dout, p->dout_cap, (size_t)total);
-  if (rc) return rc;
+  // the plan holds buffers for the largest record of its dimension
+  if (need_part > p->dpart_cap || need_out > p->dout_cap || !p->d_dpart || !p->d_dout)
+    return fail(FSX_ERR_ARG, "diagnostics buffers of the plan too small");
   hipStream_t s = (hipStream_t)stream;
   const Box box = make_box(p, false);
   for (int q0 = 0; q0 < Q; q0 += qstep) {

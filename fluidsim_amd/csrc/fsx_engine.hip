@@ -28,12 +28,13 @@ int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
   if (dim == 2) nz = 1;
   if (precision != 4 && precision != 8)
     return fail(FSX_ERR_ARG, "precision must be 4 (float32) or 8 (float64)");
-  if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz) || nx < 4 || ny < 2 ||
-      (dim == 3 && nz < 2))
-    return fail(FSX_ERR_ARG,
-                "the engine supports power-of-two grids (nx >= 4, ny, nz >= 2)");
+  if (nx < 2 || ny < 2 || (dim == 3 && nz < 2))
+    return fail(FSX_ERR_ARG, "grid dimensions must be >= 2");
+  // power-of-two axes: register/LDS Stockham passes; other lengths:
+  // mixed-radix passes with two LDS tile buffers (half the length)
   const int tile = (precision == 4 ? kEPT32 : kEPT64) * kThr;
-  if (ny > tile || nz > tile || nx / 2 > tile)
+  auto axis_max = [&](int n) { return is_pow2(n) ? tile : tile / 2; };
+  if (ny > axis_max(ny) || nz > axis_max(nz) || (is_pow2(nx) ? nx / 2 > tile : nx > tile / 2))
     return fail(FSX_ERR_ARG, "grid dimension too large for the LDS tile");
   if (nf_max < 1 || nf_max > 8) return fail(FSX_ERR_ARG, "nf_max must be 1..8");
   if (!(Lx > 0) || !(Ly > 0) || (dim == 3 && !(Lz > 0)))
@@ -131,6 +132,10 @@ int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
   if (e != hipSuccess) {
     fsx_plan_destroy(p);
     return fail(FSX_ERR_HIP, std::string("workspace: ") + hipGetErrorString(e));
+  }
+  if ((rc = fsx_diag_alloc(p))) {
+    fsx_plan_destroy(p);
+    return rc;
   }
   *out = p;
   return 0;

@@ -131,6 +131,41 @@ __device__ __forceinline__ void dftR(cpx<T>* v) {
     dft2<T, DIR>(v);
 }
 
+// 3-point DFT: y0 = a + b + c, y1,2 = a - (b + c)/2 +- i DIR (sqrt 3 / 2)(b - c).
+template <typename T, int DIR>
+__device__ __forceinline__ void dft3(cpx<T>* v) {
+  const T h = T(0.86602540378443864676372317075293618347);
+  const cpx<T> s = v[1] + v[2];
+  const cpx<T> t = v[0] - scale(s, T(0.5));
+  const cpx<T> u = rot90<T, DIR>(scale(v[1] - v[2], h));
+  v[0] = v[0] + s;
+  v[1] = t + u;
+  v[2] = t - u;
+}
+
+// 5-point DFT from the cosines and sines of 2 pi / 5 and 4 pi / 5:
+// y1,4 = x0 + c1 a1 + c2 a2 +- i DIR (s1 b1 + s2 b2),
+// y2,3 = x0 + c2 a1 + c1 a2 +- i DIR (s2 b1 - s1 b2),
+// a1 = x1 + x4, a2 = x2 + x3, b1 = x1 - x4, b2 = x2 - x3.
+template <typename T, int DIR>
+__device__ __forceinline__ void dft5(cpx<T>* v) {
+  const T c1 = T(0.30901699437494742410229341718281905886);
+  const T c2 = T(-0.80901699437494742410229341718281905886);
+  const T s1 = T(0.95105651629515357211643933337938214340);
+  const T s2 = T(0.58778525229247312916870595463907276860);
+  const cpx<T> a1 = v[1] + v[4], a2 = v[2] + v[3];
+  const cpx<T> b1 = v[1] - v[4], b2 = v[2] - v[3];
+  const cpx<T> t1 = v[0] + scale(a1, c1) + scale(a2, c2);
+  const cpx<T> t2 = v[0] + scale(a1, c2) + scale(a2, c1);
+  const cpx<T> u1 = rot90<T, DIR>(scale(b1, s1) + scale(b2, s2));
+  const cpx<T> u2 = rot90<T, DIR>(scale(b1, s2) - scale(b2, s1));
+  v[0] = v[0] + a1 + a2;
+  v[1] = t1 + u1;
+  v[4] = t1 - u1;
+  v[2] = t2 + u2;
+  v[3] = t2 - u2;
+}
+
 // Radix of stage s of a length-2^logN transform (8s first, then 4 or 2).
 __host__ __device__ __forceinline__ int stage_log_radix(int logN, int logNs) {
   const int rem = logN - logNs;
@@ -255,6 +290,152 @@ __device__ __forceinline__ void lds_fft_rows(cpx<T>* buf, int logN, int lines,
       lds_stage_rows<T, DIR, EPT, 2, kT>(buf, logN, logNs, lines, ldr, tw, twshift);
     logNs += lr;
   }
+}
+
+// ------------------------------------------------------ mixed-radix lengths
+// Transforms of any length N (grids that are not powers of two) run as
+// Stockham stages of radix 8, 4, 2, 3, 5 and, for any other prime factor p,
+// a direct p-point DFT.  Every stage reads one LDS buffer and writes the
+// other (no register stash across the barrier), with the same index rule as
+// the power-of-two stages: butterfly j (k = j mod Ns) reads positions
+// j + r N/R, applies W_{Ns R}^{r k} and DFT_R, and writes output r to
+// (j - k) R + k + r Ns.  All twiddles are entries of the length-N table
+// tw[m] = exp(-2 pi i m / N).
+constexpr int kMaxStages = 16;
+
+// Element (line, pos) of a tile: line-fast (strided passes, `lines` a power
+// of two) or row layout (row passes, row stride ld).
+struct TileLayout {
+  int logL;  // line-fast: pos << logL | line
+  int ld;    // row layout (logL < 0): line * ld + pos
+  __device__ __forceinline__ int at(int line, int pos) const {
+    return logL >= 0 ? lpad((pos << logL) + line) : line * ld + pos;
+  }
+};
+
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void gen_butterfly(const cpx<T>* A, cpx<T>* B,
+                                              const TileLayout& lay, const cpx<T>* tw,
+                                              int N, int Ns, int line, int j) {
+  const int NR = N / R;
+  const int k = j % Ns;
+  const int step = NR / Ns;  // N / (Ns R)
+  cpx<T> v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    v[r] = A[lay.at(line, j + r * NR)];
+    if (r > 0 && k > 0) {
+      cpx<T> w = tw[r * k * step];
+      if (DIR > 0) w.y = -w.y;
+      v[r] = v[r] * w;
+    }
+  }
+  if constexpr (R == 3)
+    dft3<T, DIR>(v);
+  else if constexpr (R == 5)
+    dft5<T, DIR>(v);
+  else
+    dftR<T, DIR, R>(v);
+  const int o = (j - k) * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) B[lay.at(line, o + r * Ns)] = v[r];
+}
+
+// Direct DFT of a prime radix R without a dedicated butterfly: output r of
+// butterfly j is sum_m (x_m W_{Ns R}^{m k}) W_R^{m r}.
+template <typename T, int DIR>
+__device__ __forceinline__ void gen_direct(const cpx<T>* A, cpx<T>* B, const TileLayout& lay,
+                                           const cpx<T>* tw, int N, int R, int Ns,
+                                           int line, int j) {
+  const int NR = N / R;
+  const int k = j % Ns;
+  const int step = NR / Ns;
+  const int o = (j - k) * R + k;
+  for (int r = 0; r < R; ++r) {
+    cpx<T> acc = zero<T>();
+    for (int m = 0; m < R; ++m) {
+      cpx<T> x = A[lay.at(line, j + m * NR)];
+      cpx<T> w = tw[m * k * step];
+      cpx<T> u = tw[((m * r) % R) * NR];
+      if (DIR > 0) {
+        w.y = -w.y;
+        u.y = -u.y;
+      }
+      acc = acc + (x * w) * u;
+    }
+    B[lay.at(line, o + r * Ns)] = acc;
+  }
+}
+
+// All stages of `lines` transforms of length N over the tile in A (B is the
+// second buffer).  The caller synchronises before; returns the buffer that
+// holds the result, after a barrier.
+template <typename T, int DIR, int kT>
+__device__ cpx<T>* gen_fft(cpx<T>* A, cpx<T>* B, const TileLayout& lay, const cpx<T>* tw,
+                           int N, int lines, int nst, const int* rad) {
+  int Ns = 1;
+  for (int s = 0; s < nst; ++s) {
+    const int R = rad[s];
+    const int nb = (N / R) * lines;
+    for (int b = threadIdx.x; b < nb; b += kT) {
+      int line, j;
+      if (lay.logL >= 0) {
+        line = b & ((1 << lay.logL) - 1);
+        j = b >> lay.logL;
+      } else {
+        line = b / (N / R);
+        j = b - line * (N / R);
+      }
+      switch (R) {
+        case 2: gen_butterfly<T, DIR, 2>(A, B, lay, tw, N, Ns, line, j); break;
+        case 3: gen_butterfly<T, DIR, 3>(A, B, lay, tw, N, Ns, line, j); break;
+        case 4: gen_butterfly<T, DIR, 4>(A, B, lay, tw, N, Ns, line, j); break;
+        case 5: gen_butterfly<T, DIR, 5>(A, B, lay, tw, N, Ns, line, j); break;
+        case 8: gen_butterfly<T, DIR, 8>(A, B, lay, tw, N, Ns, line, j); break;
+        default: gen_direct<T, DIR>(A, B, lay, tw, N, R, Ns, line, j); break;
+      }
+    }
+    __syncthreads();
+    cpx<T>* t = A;
+    A = B;
+    B = t;
+    Ns *= R;
+  }
+  return A;
+}
+
+// Stage radices of a length-n transform (host): 8s, then 4 or 2, then 3s,
+// 5s and the remaining primes in increasing order.  Returns the number of
+// stages, or -1 when more than kMaxStages would be needed.
+inline int gen_factor(int n, int* rad) {
+  int ns = 0;
+  int p2 = 0;
+  while (n % 2 == 0) {
+    n /= 2;
+    ++p2;
+  }
+  while (p2 >= 3) {
+    if (ns >= kMaxStages) return -1;
+    rad[ns++] = 8;
+    p2 -= 3;
+  }
+  if (p2 > 0) {
+    if (ns >= kMaxStages) return -1;
+    rad[ns++] = 1 << p2;
+  }
+  for (int p = 3; n > 1; p += 2) {
+    while (n % p == 0) {
+      if (ns >= kMaxStages) return -1;
+      rad[ns++] = p;
+      n /= p;
+    }
+    if ((long long)p * p > n && n > 1) {
+      if (ns >= kMaxStages) return -1;
+      rad[ns++] = n;
+      n = 1;
+    }
+  }
+  return ns;
 }
 
 // Kept-index map of a signed (y or z) wavenumber axis: kept indices are

@@ -30,6 +30,7 @@ using namespace fsx;
 // State shared by the translation units of the engine (defined in
 // fsx_engine.hip).
 struct fsx_plan_s;
+int fsx_diag_alloc(fsx_plan_s* p);  // fsx_api_diag.hip
 extern thread_local std::string fsx_g_err;      // message of the last failure
 extern thread_local fsx_plan_s* fsx_g_plan;     // plan of the running API call
 extern thread_local int fsx_g_kind;             // timing kind of the next launch
@@ -237,12 +238,13 @@ struct line_mul<Op, std::void_t<decltype(Op::kLineMul)>> {
 
 // LDS of a strided tile of `tile` elements (N positions x its lines; the
 // kernel touches only the padded tile after the N twiddles)
-template <typename T, int DIR, int E, int KT, class Op>
+template <typename T, int DIR, int E, int KT, class Op, bool GEN = false>
 int launch_strided_tile(Op& op, int N, size_t tile_max, long long blocks, int nfb,
                         hipStream_t s) {
   const size_t tile = std::min(tile_max, (size_t)E * KT);
-  const size_t lds = (tile + tile / 32 + N) * sizeof(cpx<T>);
-  auto kern = k_strided<T, DIR, E, KT, Op>;
+  // mixed-radix lengths: two tile buffers (gen_fft stages are out of place)
+  const size_t lds = ((GEN ? 2 : 1) * (tile + tile / 32 + 1) + N) * sizeof(cpx<T>);
+  auto kern = k_strided<T, DIR, E, KT, Op, GEN>;
   static size_t lds_set = 0;
   if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
   if (lds > lds_set) {
@@ -271,9 +273,21 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   // wider plain y-passes at 8 lines are slower)
   constexpr int KW = 2 * kThr;
   constexpr int wide_below = Op::kEptDiv > 1 ? (int)(128 / sizeof(cpx<T>)) : 8;
-  const bool wide_ok = sizeof(T) == 4 && tile / N < wide_below &&
+  // lengths that are not powers of two: mixed-radix stages in LDS, 512-thread
+  // tiles of at most half the size (two buffers)
+  const bool gen = !is_pow2(N);
+  const bool wide_ok = !gen && sizeof(T) == 4 && tile / N < wide_below &&
                        ((size_t)E * KW + E * KW / 32 + N) * sizeof(cpx<T>) <= kLdsMax;
   int L = (wide_ok ? E * KW : tile) / N;
+  if (gen) {
+    op.d.nst = gen_factor(N, op.d.rad);
+    if (op.d.nst < 0) return fail(FSX_ERR_ARG, "transform length has too many factors");
+    L = (tile / 2) / N;
+    if (L < 1) return fail(FSX_ERR_ARG, "transform length exceeds the tile");
+  } else {
+    op.d.nst = 0;
+  }
+  while (L & (L - 1)) L &= L - 1;  // power of two
   int cap = 1;
   while (cap < nlines) cap <<= 1;
   if (L > cap) L = cap;
@@ -307,6 +321,11 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   // elements the op transforms at once: N L, or N L times its line
   // multiple when that fits a kThr tile
   size_t need = (size_t)N * L;
+  if (gen) {
+    // the same test as the op's own (stacked lines fit the kThr tile)
+    if (need * line_mul<Op>::value <= (size_t)E * kThr) need *= line_mul<Op>::value;
+    return launch_strided_tile<T, DIR, E, kThr, Op, true>(op, N, need, blocks, nfb, s);
+  }
   if (!wide && need * line_mul<Op>::value <= (size_t)E * kThr) need *= line_mul<Op>::value;
   if (wide) return launch_strided_tile<T, DIR, E, KW>(op, N, need, blocks, nfb, s);
   // small problems (few lines per tile): as many threads as the tile needs
@@ -317,10 +336,65 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
   return launch_strided_tile<T, DIR, E, kThr>(op, N, need, blocks, nfb, s);
 }
 
+// Row pass of a length nx that is not a power of two (k_xpass_gen).
+template <typename T, class Op>
+int launch_xpass_gen(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
+                     int ld_in, int mx_out, int ld_out, long long ifs, long long ofs,
+                     T scl, const void* tw, hipStream_t s) {
+  constexpr int E = ept<T>() / FSX_XPASS_EPT_DIV >= 8 ? ept<T>() / FSX_XPASS_EPT_DIV : 8;
+  (void)E;
+  // two buffers of complex rows in the workgroup's LDS, after the twiddles
+  const int tile = (int)((kLdsMax / sizeof(cpx<T>) - (size_t)nx) / 2);
+  const int nfm = nf_in > nf_out ? nf_in : nf_out;
+  const int ld = nx + 1;
+  if (nfm * ld > tile) return fail(FSX_ERR_ARG, "row pass exceeds the tile");
+  int RB = tile / (nfm * ld);
+  if (RB > nrows) RB = nrows > 0 ? nrows : 1;
+  while (RB > 1 && (nrows + RB - 1) / RB < kMinBlocks) RB = (RB + 1) / 2;
+  op.d.nx = nx;
+  op.d.lognx = -1;
+  op.d.M = 0;
+  op.d.logM = -1;
+  op.d.RB = RB;
+  op.d.logRB = -1;
+  op.d.nrows = nrows;
+  op.d.nf_in = nf_in;
+  op.d.nf_out = nf_out;
+  op.d.mx_in = mx_in;
+  op.d.ld_in = ld_in;
+  op.d.mx_out = mx_out;
+  op.d.ld_out = ld_out;
+  op.d.ifs = ifs;
+  op.d.ofs = ofs;
+  op.d.scale = scl;
+  op.d.tw = reinterpret_cast<const cpx<T>*>(tw);
+  op.d.nst = gen_factor(nx, op.d.rad);
+  if (op.d.nst < 0) return fail(FSX_ERR_ARG, "row length has too many factors");
+  if (nrows <= 0) return 0;
+  const long long blocks = (nrows + RB - 1) / RB;
+  const size_t lds = ((size_t)2 * nfm * RB * ld + nx) * sizeof(cpx<T>);
+  auto kern = k_xpass_gen<T, kXThr, Op>;
+  static size_t lds_set = 0;
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "pass needs more LDS than a workgroup has");
+  if (lds > lds_set) {
+    int rc = set_lds(kern, lds);
+    if (rc) return rc;
+    lds_set = lds;
+  }
+  int rc = tm_pre(s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kXThr), lds, s, op);
+  FSX_HIP(hipGetLastError());
+  return tm_post(s);
+}
+
 template <typename T, class Op>
 int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
                  int ld_in, int mx_out, int ld_out, long long ifs,
                  long long ofs, T scl, const void* tw, hipStream_t s) {
+  if (!is_pow2(nx) || nx < 4)
+    return launch_xpass_gen<T>(op, nx, nrows, nf_in, nf_out, mx_in, ld_in, mx_out, ld_out,
+                               ifs, ofs, scl, tw, s);
   constexpr int E = ept<T>() / FSX_XPASS_EPT_DIV >= 8 ? ept<T>() / FSX_XPASS_EPT_DIV : 8;
   const int tile = E * kXThr;
   const int M = nx / 2;
@@ -427,7 +501,7 @@ int launch_xrow(int nx, int nrows, int mx_in, int ld_in, int mx_out, int ld_out,
                 long long ifs, long long ofs, T scl, const void* in, void* out,
                 const void* tw, unsigned long long* maxima, const Op& op,
                 hipStream_t s) {
-  if (!FSX_XROW) return 1;
+  if (!FSX_XROW || !is_pow2(nx)) return 1;
   XRDesc<T> d;
   d.nrows = nrows;
   d.mx_in = mx_in;

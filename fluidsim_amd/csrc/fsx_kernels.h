@@ -34,6 +34,9 @@ struct StridedDesc {
   int nf;         // fields transformed one after the other
   int kio;        // tiles remapped per XCD (K rows, or runs under a cache line)
   const cpx<T>* tw;  // exp(-2 pi i m / N), m < N (copied to LDS)
+  // mixed-radix stages of a length that is not a power of two (gen_fft)
+  int nst;
+  int rad[kMaxStages];
 };
 
 // Full transform of one tile.  src(e, l, p) gives the input at position p
@@ -41,9 +44,9 @@ struct StridedDesc {
 // calls); dst(e, l, p, v) receives the output at position p (e = register
 // slot of the last stage).  Ends with a barrier so buf can be reused.
 template <typename T, int DIR, int EPT, int kT, class Src, class Dst>
-__device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
-                                         int logN, int logL, Src&& src,
-                                         Dst&& dst) {
+__device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
+                                            int logN, int logL, Src&& src,
+                                            Dst&& dst) {
   const int lmask = (1 << logL) - 1;
   if (logN <= 3) {
     // single stage: HBM -> registers -> HBM
@@ -131,11 +134,51 @@ __device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
   __syncthreads();
 }
 
+// Full transform of one tile of 2^logL lines of length d.N.  GEN = false:
+// power-of-two length (tile_fft_p2, registers at both ends).  GEN = true:
+// any length (gen_fft); slot e of both src and dst is the tile element
+// threadIdx.x + e kT (line-fast), and the tile uses two LDS buffers.
+template <typename T, int DIR, int EPT, int kT, bool GEN, class Src, class Dst>
+__device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
+                                         const StridedDesc<T>& d, int logL, Src&& src,
+                                         Dst&& dst) {
+  if constexpr (!GEN) {
+    tile_fft_p2<T, DIR, EPT, kT>(buf, tw, d.logN, logL, src, dst);
+  } else {
+    const int tot = d.N << logL;
+    const int lmask = (1 << logL) - 1;
+    cpx<T>* A = buf;
+    cpx<T>* B = buf + tot + (tot >> 5) + 1;
+    const TileLayout lay{logL, 0};
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = threadIdx.x + e * kT;
+      if (idx < tot) A[lpad(idx)] = src(e, idx & lmask, idx >> logL);
+    }
+    __syncthreads();
+    const cpx<T>* res = gen_fft<T, DIR, kT>(A, B, lay, tw, d.N, 1 << logL, d.nst, d.rad);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = threadIdx.x + e * kT;
+      if (idx < tot) dst(e, idx & lmask, idx >> logL, res[lpad(idx)]);
+    }
+    __syncthreads();
+  }
+}
+
 // Position of register slot e of tile_fft's last stage: returns false when
 // the slot holds no butterfly output, else its local line and position.
-template <int EPT, int kT>
-__device__ __forceinline__ bool last_slot(int e, int logN, int logL, int& line,
-                                          int& p) {
+template <int EPT, int kT, bool GEN, typename T>
+__device__ __forceinline__ bool last_slot(int e, const StridedDesc<T>& d, int logL,
+                                          int& line, int& p) {
+  if constexpr (GEN) {
+    const int idx = threadIdx.x + e * kT;
+    if (idx >= (d.N << logL)) return false;
+    line = idx & ((1 << logL) - 1);
+    p = idx >> logL;
+    return true;
+  }
+  const int logN = d.logN;
   int logR;
   if (logN <= 3) {
     logR = logN;
@@ -194,7 +237,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 #define FSX_STRIDED_ATTR
 #endif
 
-template <typename T, int DIR, int EPT, int kT, class Op>
+template <typename T, int DIR, int EPT, int kT, class Op, bool GEN = false>
 static __global__ __launch_bounds__(kT) FSX_STRIDED_ATTR void k_strided(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const StridedDesc<T>& d = op.d;
@@ -206,7 +249,7 @@ static __global__ __launch_bounds__(kT) FSX_STRIDED_ATTR void k_strided(Op op) {
   const int b = d.kio ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int o = b / nchunks;
   const int line0 = (b - o * nchunks) << d.logL;
-  op.template run<DIR, EPT, kT>(buf, s_tw, o, line0);
+  op.template run<DIR, EPT, kT, GEN>(buf, s_tw, o, line0);
 }
 
 // Plain pass: one field per workgroup (blockIdx.y), load()/store().
@@ -214,13 +257,13 @@ template <typename T, class Self>
 struct PlainRun {
   static constexpr int kEptDiv = 1;
   static constexpr bool kFieldPerBlock = true;
-  template <int DIR, int EPT, int kT>
+  template <int DIR, int EPT, int kT, bool GEN>
   __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const Self& s = static_cast<const Self&>(*this);
     const int nlines = s.d.nlines;
     const int f = blockIdx.y;
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, s.d.logN, s.d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, s.d, s.d.logL,
         [&](int, int l, int p) {
           const int line = line0 + l;
           return line < nlines ? s.load(f, o, line, p) : zero<T>();
@@ -371,7 +414,7 @@ struct OpZInvCurl {
   T fcor;
   int with_b;  // ns3d.strat: input 3 (b) transformed into output 6
 
-  template <int DIR, int EPT, int kT>
+  template <int DIR, int EPT, int kT, bool GEN>
   __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
     const int iy = ym.full(o);
@@ -389,15 +432,15 @@ struct OpZInvCurl {
     };
     auto kz_of = [&](int p) { return T(ksigned(p, nz)) * dkz; };
     // Vz
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
         [&](int e, int l, int p, cpx<T> v) {
           sVz[e] = v;
           st(2, l, p, v);
         });
     // omega_x = i ky Vz - Dz[vy]
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int e, int l, int p) {
           const cpx<T> a = ld(1, l, p);
           sv[e] = a;
@@ -407,15 +450,15 @@ struct OpZInvCurl {
           st(3, l, p, mul_i(scale(sVz[e], ky)) - v);
         });
     // Vy
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int e, int, int) { return sv[e]; },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int e, int, int) { return sv[e]; },
         [&](int e, int l, int p, cpx<T> v) {
           sVy[e] = v;
           st(1, l, p, v);
         });
     // omega_y = Dz[vx] - i kx Vz
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int e, int l, int p) {
           const cpx<T> a = ld(0, l, p);
           sv[e] = a;
@@ -426,8 +469,8 @@ struct OpZInvCurl {
           st(4, l, p, v - mul_i(scale(sVz[e], kx)));
         });
     // Vx and omega_z = i kx Vy - i ky Vx (+ f)
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int e, int, int) { return sv[e]; },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int e, int, int) { return sv[e]; },
         [&](int e, int l, int p, cpx<T> v) {
           const T kx = T(line0 + l) * dkx;
           st(0, l, p, v);
@@ -436,8 +479,8 @@ struct OpZInvCurl {
           st(5, l, p, w);
         });
     if (with_b)
-      tile_fft<T, DIR, EPT, kT>(
-          buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
+      tile_fft<T, DIR, EPT, kT, GEN>(
+          buf, tw, d, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
           [&](int, int l, int p, cpx<T> v) { st(6, l, p, v); });
   }
 };
@@ -461,7 +504,7 @@ struct OpYInv2NS {
   AxisMap ym;
   T dkx, dky;
 
-  template <int DIR, int EPT, int kT>
+  template <int DIR, int EPT, int kT, bool GEN>
   __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
     const cpx<T>* in = this->in + o * ips;
@@ -481,8 +524,8 @@ struct OpYInv2NS {
       // one transform of 4 L lines: line l is output l >> logL of line
       // l & (L - 1), each line computed as in the sequential form below
       const int lm = (1 << d.logL) - 1;
-      tile_fft<T, DIR, EPT, kT>(
-          buf, tw, d.logN, d.logL + 2,
+      tile_fft<T, DIR, EPT, kT, GEN>(
+          buf, tw, d, d.logL + 2,
           [&](int, int l, int p) {
             const int c = l >> d.logL, li = l & lm;
             const int line = line0 + li;
@@ -502,8 +545,8 @@ struct OpYInv2NS {
           [&](int, int l, int p, cpx<T> v) { st(l >> d.logL, l & lm, p, v); });
       return;
     }
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int e, int l, int p) {
           const int line = line0 + l;
           cpx<T> r = zero<T>();
@@ -513,19 +556,19 @@ struct OpYInv2NS {
           return mul_i(scale(r, ky * inv_k2(l, p)));
         },
         [&](int, int l, int p, cpx<T> v) { st(0, l, p, v); });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int e, int l, int p) {
           const T kx = T(line0 + l) * dkx;
           return mul_i(scale(sr[e], -kx * inv_k2(l, p)));
         },
         [&](int, int l, int p, cpx<T> v) { st(1, l, p, v); });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int e, int l, int) { return mul_i(scale(sr[e], T(line0 + l) * dkx)); },
         [&](int, int l, int p, cpx<T> v) { st(2, l, p, v); });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int e, int, int p) { return mul_i(scale(sr[e], T(ksigned(p, ny)) * dky)); },
         [&](int, int l, int p, cpx<T> v) { st(3, l, p, v); });
   }
@@ -547,7 +590,7 @@ struct OpZFwdRK3 {
   const unsigned char* mask;
   RKArgs<T> rk;
 
-  template <int DIR, int EPT, int kT>
+  template <int DIR, int EPT, int kT, bool GEN>
   __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
     const int nyk = ym.nk();
@@ -560,19 +603,19 @@ struct OpZFwdRK3 {
       if (line >= nlines) return zero<T>();
       return in[c * ifs + (p * nyk + o) * ld1 + line];
     };
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
         [&](int e, int, int, cpx<T> v) { s0[e] = v; });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
         [&](int e, int, int, cpx<T> v) { s1[e] = v; });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
         [&](int e, int, int, cpx<T> v) { s2[e] = v; });
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       int l, p;
-      if (!last_slot<EPT, kT>(e, d.logN, d.logL, l, p)) continue;
+      if (!last_slot<EPT, kT, GEN>(e, d, d.logL, l, p)) continue;
       const int line = line0 + l;
       if (line >= nlines || zm.compact(p) < 0) continue;
       const int idx = (p * ny + iy) * nkx + line;
@@ -603,7 +646,7 @@ struct OpZFwdRKStrat {
   T N2;
   RKArgs<T> rk;
 
-  template <int DIR, int EPT, int kT>
+  template <int DIR, int EPT, int kT, bool GEN>
   __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
     const int nyk = ym.nk();
@@ -617,31 +660,31 @@ struct OpZFwdRKStrat {
       if (line >= nlines) return zero<T>();
       return in[c * ifs + (p * nyk + o) * ld1 + line];
     };
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
         [&](int e, int, int, cpx<T> v) { s0[e] = v; });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
         [&](int e, int, int, cpx<T> v) { s1[e] = v; });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
         [&](int e, int, int, cpx<T> v) { s2[e] = v; });
     // sb = kx G_x + ky G_y + kz G_z  (N_b = -i sb - N^2 v_z)
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
         [&](int e, int l, int, cpx<T> v) { sb[e] = scale(v, T(line0 + l) * dkx); });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(4, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(4, l, p); },
         [&](int e, int, int, cpx<T> v) { sb[e] = sb[e] + scale(v, ky); });
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL, [&](int, int l, int p) { return ld(5, l, p); },
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(5, l, p); },
         [&](int e, int, int p, cpx<T> v) {
           sb[e] = sb[e] + scale(v, T(ksigned(p, d.N)) * dkz);
         });
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       int l, p;
-      if (!last_slot<EPT, kT>(e, d.logN, d.logL, l, p)) continue;
+      if (!last_slot<EPT, kT, GEN>(e, d, d.logL, l, p)) continue;
       const int line = line0 + l;
       if (line >= nlines || zm.compact(p) < 0) continue;
       const int idx = (p * ny + iy) * nkx + line;
@@ -671,7 +714,7 @@ struct OpYFwd2RK {
   long long ips = 0, ops = 0;
   const double* bpar = nullptr;
 
-  template <int DIR, int EPT, int kT>
+  template <int DIR, int EPT, int kT, bool GEN>
   __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
     const int nlines = d.nlines;
     const cpx<T>* in = this->in + o * ips;
@@ -686,8 +729,8 @@ struct OpYFwd2RK {
       visc = rk.visc;
     }
     const size_t base = (size_t)o * ops;
-    tile_fft<T, DIR, EPT, kT>(
-        buf, tw, d.logN, d.logL,
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
         [&](int, int l, int p) {
           const int line = line0 + l;
           return line < nlines ? in[p * ld2 + line] : zero<T>();
@@ -715,6 +758,9 @@ struct XDesc {
   long long ifs, ofs;  // field strides (elements)
   T scale;             // applied to the r2c output
   const cpx<T>* tw;    // exp(-2 pi i m / nx), m < nx (copied to LDS)
+  // mixed-radix stages of a length-nx complex transform (k_xpass_gen)
+  int nst;
+  int rad[kMaxStages];
 };
 
 template <typename T>
@@ -867,10 +913,89 @@ void k_xpass(Op op) {
   }
 }
 
+// Row pass of any length nx (grids that are not powers of two): each real
+// row is one length-nx complex transform in LDS (gen_fft, row layout with
+// stride nx + 1).  The c2r input is the half spectrum extended by Hermitian
+// symmetry, X[nx - k] = conj X[k]; the real part of the inverse transform is
+// the row (the imaginary parts of X[0] and, for even nx, X[nx/2] drop out,
+// as in numpy's irfft).  The products run per point (Op::point) on the real
+// parts; the forward transform of the real product rows keeps kx < mx_out.
+// Rows per workgroup: d.RB (any count), fields stacked as rows f RB + r.
+template <typename T, int kT, class Op>
+static __global__ __launch_bounds__(kT) void k_xpass_gen(Op op) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const XDesc<T>& d = op.d;
+  const int nx = d.nx, ld = nx + 1, RB = d.RB;
+  cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int nfm = d.nf_in > d.nf_out ? d.nf_in : d.nf_out;
+  cpx<T>* A = s_tw + nx;
+  cpx<T>* B = A + nfm * RB * ld;
+  const int row0 = blockIdx.x * RB;
+  const int tid = threadIdx.x;
+  const TileLayout lay{-1, ld};
+  for (int i = tid; i < nx; i += kT) s_tw[i] = d.tw[i];
+  const int n_in = d.nf_in * RB * nx;
+  for (int idx = tid; idx < n_in; idx += kT) {
+    const int lr = idx / nx, x = idx - lr * nx;
+    const int f = lr / RB, row = row0 + (lr - f * RB);
+    cpx<T> v = zero<T>();
+    if (row < d.nrows) {
+      if constexpr (Op::kInv) {
+        const int k = x <= nx / 2 ? x : nx - x;
+        if (k < d.mx_in) v = op.in[f * d.ifs + (long long)row * d.ld_in + k];
+        if (x > nx / 2) v = conj(v);
+      } else {
+        v.x = op.inr[f * d.ifs + (long long)row * nx + x];
+      }
+    }
+    A[lr * ld + x] = v;
+  }
+  __syncthreads();
+  cpx<T>* res = A;
+  if constexpr (Op::kInv) {
+    res = gen_fft<T, +1, kT>(A, B, lay, s_tw, nx, d.nf_in * RB, d.nst, d.rad);
+    if constexpr (Op::kStoreReal) {
+      for (int idx = tid; idx < n_in; idx += kT) {
+        const int lr = idx / nx, x = idx - lr * nx;
+        const int f = lr / RB, row = row0 + (lr - f * RB);
+        if (row < d.nrows) op.outr[f * d.ofs + (long long)row * nx + x] = res[lr * ld + x].x;
+      }
+      return;
+    } else {
+      T m[3] = {0, 0, 0};
+      for (int idx = tid; idx < RB * nx; idx += kT) {
+        const int r = idx / nx, x = idx - r * nx;
+        T v[8];
+        for (int c = 0; c < d.nf_in && c < 8; ++c) v[c] = res[(c * RB + r) * ld + x].x;
+        T mm[3] = {0, 0, 0};
+        op.point(v, mm);
+        if (row0 + r < d.nrows)
+          for (int c = 0; c < 3; ++c) m[c] = fmax(m[c], mm[c]);
+        for (int c = 0; c < d.nf_out; ++c) res[(c * RB + r) * ld + x] = mk<T>(v[c], T(0));
+      }
+      if constexpr (Op::kNMax > 0)
+        if (op.maxima) block_max_atomic<T, kT>(m[0], m[1], m[2], Op::kNMax, op.maxima);
+      __syncthreads();
+    }
+  }
+  cpx<T>* other = res == A ? B : A;
+  res = gen_fft<T, -1, kT>(res, other, lay, s_tw, nx, d.nf_out * RB, d.nst, d.rad);
+  const int n_out = d.nf_out * RB * d.mx_out;
+  for (int idx = tid; idx < n_out; idx += kT) {
+    const int lr = idx / d.mx_out, k = idx - lr * d.mx_out;
+    const int f = lr / RB, row = row0 + (lr - f * RB);
+    if (row < d.nrows)
+      op.out[f * d.ofs + (long long)row * d.ld_out + k] = scale(res[lr * ld + k], d.scale);
+  }
+}
+
 // plain c2r rows: W2 rows -> X rows
 template <typename T>
 struct OpXC2R {
   static constexpr bool kInv = true, kStoreReal = true;
+  static constexpr int kNMax = 0;
+  unsigned long long* maxima = nullptr;
+  __device__ void point(T*, T*) const {}
   XDesc<T> d;
   const cpx<T>* in;
   const T* inr;
@@ -883,6 +1008,9 @@ struct OpXC2R {
 template <typename T>
 struct OpXR2C {
   static constexpr bool kInv = false, kStoreReal = false;
+  static constexpr int kNMax = 0;
+  unsigned long long* maxima = nullptr;
+  __device__ void point(T*, T*) const {}
   XDesc<T> d;
   const cpx<T>* in;
   const T* inr;
@@ -901,6 +1029,17 @@ struct OpXNS3D {
   T* outr;
   cpx<T>* out;
   unsigned long long* maxima;  // 3 slots or null
+  static constexpr int kNMax = 3;
+  // one point: v = (v, omega) -> v x omega, maxima of |v_i| into m
+  __device__ void point(T* v, T* m) const {
+    const T vx = v[0], vy = v[1], vz = v[2];
+    v[0] = vy * v[5] - vz * v[4];
+    v[1] = vz * v[3] - vx * v[5];
+    v[2] = vx * v[4] - vy * v[3];
+    m[0] = fmax(m[0], fabs(vx));
+    m[1] = fmax(m[1], fabs(vy));
+    m[2] = fmax(m[2], fabs(vz));
+  }
   __device__ void physical(T* b, int row0, int ldr) const {
     const int n = d.RB << d.lognx;
     const int s = 2 * ldr * d.RB;  // reals per field
@@ -932,6 +1071,20 @@ struct OpXStrat {
   T* outr;
   cpx<T>* out;
   unsigned long long* maxima;  // 3 slots or null
+  static constexpr int kNMax = 3;
+  // one point: (v, omega, b) -> (v x omega, v b), maxima of |v_i| into m
+  __device__ void point(T* v, T* m) const {
+    const T vx = v[0], vy = v[1], vz = v[2], bb = v[6];
+    v[0] = vy * v[5] - vz * v[4];
+    v[1] = vz * v[3] - vx * v[5];
+    v[2] = vx * v[4] - vy * v[3];
+    v[3] = vx * bb;
+    v[4] = vy * bb;
+    v[5] = vz * bb;
+    m[0] = fmax(m[0], fabs(vx));
+    m[1] = fmax(m[1], fabs(vy));
+    m[2] = fmax(m[2], fabs(vz));
+  }
   __device__ void physical(T* b, int row0, int ldr) const {
     const int n = d.RB << d.lognx;
     const int s = 2 * ldr * d.RB;
@@ -967,6 +1120,14 @@ struct OpXNS2D {
   cpx<T>* out;
   unsigned long long* maxima;  // 2 slots or null
   T beta;
+  static constexpr int kNMax = 2;
+  // one point: (ux, uy, d_x rot, d_y rot) -> -ux d_x rot - uy (d_y rot + beta)
+  __device__ void point(T* v, T* m) const {
+    const T ux = v[0], uy = v[1], px = v[2], py = v[3];
+    v[0] = beta == T(0) ? -ux * px - uy * py : -ux * px - uy * (py + beta);
+    m[0] = fmax(m[0], fabs(ux));
+    m[1] = fmax(m[1], fabs(uy));
+  }
   __device__ void physical(T* b, int row0, int ldr) const {
     const int n = d.RB << d.lognx;
     const int s = 2 * ldr * d.RB;

@@ -115,6 +115,8 @@ class EnsembleNS2D:
             for pos, i in enumerate(idx):
                 self.where[i] = (g, pos)
             self.groups.append(g)
+        # the groups' buffers were zero-filled on the current stream
+        self.stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(self.stream):
             for g in self.groups:
                 self._load_group(g, cfl_coef, deltat_max)
@@ -182,6 +184,8 @@ class EnsembleNS2D:
 
     def step(self, nsteps=1, graph=True):
         """Advance every run by nsteps steps (one graph launch per step)."""
+        if nsteps <= 0:
+            return
         if graph and self._graph is None:
             # one eager step first: first launches set kernel attributes
             self._issue_step()
@@ -223,8 +227,8 @@ class EnsembleNS2D:
 
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        S0 = g.S.clone()
         with torch.cuda.stream(self.stream):
+            S0 = g.S.clone()
             for stage in range(self.nstages):  # warm
                 _lib.call("fsx_ns2d_batch_rk_stage", g.handle, self.nstages, stage,
                           _lib.dptr(g.S), _lib.dptr(g.tmp), _lib.dptr(g.t1), self.beta,
@@ -236,8 +240,8 @@ class EnsembleNS2D:
                               _lib.dptr(g.S), _lib.dptr(g.tmp), _lib.dptr(g.t1),
                               self.beta, self._sp)
             e1.record(self.stream)
+            g.S.copy_(S0)
         self.stream.synchronize()
-        g.S.copy_(S0)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 

@@ -71,6 +71,19 @@ class Parameters:
             d[t] = getattr(self, t)._make_dict_tree()
         return d
 
+    def _update_from_dict_tree(self, tree):
+        """Set attributes and children from a tree of _make_dict_tree (dict
+        values are children; attributes and children missing here are
+        created)."""
+        for k, v in tree.items():
+            if isinstance(v, dict):
+                self._set_child(k)._update_from_dict_tree(v)
+            elif k in self._tag_children:
+                raise ValueError(f"{k} is a child of {self._tag} in this tree")
+            else:
+                self._set_attrib(k, v)
+        return self
+
     def __deepcopy__(self, memo):
         new = Parameters(tag=self._tag, doc=self._doc)
         for k in self._key_attribs:

@@ -6,7 +6,7 @@ from ...base.init_fields import InitFieldsBase
 
 
 class InitFieldsNS2D(InitFieldsBase):
-    available_types = ("in_script", "constant", "noise")
+    available_types = ("in_script", "constant", "from_simul", "from_file", "noise")
 
     @classmethod
     def _complete_params_specific(cls, params):

@@ -33,7 +33,7 @@ def compute_solenoidal_noise_fft(oper, length=None, velo_max=1.0, seed=42):
 
 
 class InitFieldsNS3D(InitFieldsBase):
-    available_types = ("in_script", "constant", "noise")
+    available_types = ("in_script", "constant", "from_simul", "from_file", "noise")
 
     @classmethod
     def _complete_params_specific(cls, params):

@@ -5,7 +5,9 @@ returns are handed to ``sim.state.init_statespect_from(...)`` the way
 fluidsim's ``init_fields.type = "in_script"`` expects
 (fluidsim/base/init_fields.py:435-443, fluidsim/base/state.py:352-383).
 
-All random numbers come from ``numpy.random.Generator(PCG64(seed))``.
+The host generators draw from ``numpy.random.Generator(PCG64(seed))``;
+``ns3d_spectrum_vv_device`` (grids too large for the host, one-off set-up
+on the GPU) draws from a seeded ``torch.Generator``.
 """
 
 import os
@@ -160,3 +162,42 @@ def nu8_for_kmax_eta(n, L=2 * np.pi, epsilon=1.0, coef_dealiasing=2.0 / 3):
     """
     kmax = coef_dealiasing * (n / 2) * (2 * np.pi / L)
     return kmax ** (-22.0 / 3.0) * epsilon ** (1.0 / 3.0)
+
+
+def ns3d_spectrum_vv_device(oper, seed=0, k_peak=4.0, energy=1.0):
+    """Device counterpart of ns3d_spectrum_vv_fft for grids too large to
+    draw on the host (1024^3): the same spectrum E(k) ~ k^4 exp(-2 (k/k_peak)^2),
+    per-mode amplitude sqrt(E(k) / (4 pi k^2)) = k exp(-(k/k_peak)^2) / sqrt(4 pi)
+    times i.i.d. complex normal phases from a seeded ``torch.Generator`` on the
+    GPU (a different stream of numbers than the host generator), projected
+    with the engine's Leray projection, dealiased, made Hermitian by an
+    ifft/fft round trip through the engine and normalised to unit energy.
+    Returns three device tensors of the operator's complex dtype."""
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    kx, ky, kz = oper._kdev()
+    real = oper.dtype_real
+    K2 = kz * kz + ky * ky + kx * kx
+    amp = torch.sqrt(K2) * torch.exp(-K2 / (k_peak * k_peak)) / np.sqrt(4 * np.pi)
+    del K2
+    shape = tuple(oper.shapeK)
+    vv = []
+    for _ in range(3):
+        re = torch.randn(shape, generator=g, device="cuda", dtype=real)
+        im = torch.randn(shape, generator=g, device="cuda", dtype=real)
+        a = torch.complex(re, im)
+        del re, im
+        a.mul_(amp)
+        vv.append(a)
+    del amp
+    oper.project_perpk3d(*vv)
+    for i in range(3):
+        oper.dealiasing(vv[i])
+        vv[i] = oper.fft(oper.ifft(vv[i]))
+    E = 0.5 * sum(oper.sum_wavenumbers(a.real * a.real + a.imag * a.imag) for a in vv)
+    fac = float(np.sqrt(energy / E))
+    for a in vv:
+        a.mul_(fac)
+    return vv

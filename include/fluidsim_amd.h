@@ -47,8 +47,11 @@ int fsx_last_error(char* buf, int len);
  * (fluidfft's where_dealiased, possibly modified by truncation_shape,
  * NO_SHEAR_MODES, NO_KY0) or NULL for no dealiasing.  precision is 4 or 8
  * (bytes per real).  nf_max is the number of fields the workspace holds
- * (4 for ns2d, 6 for ns3d, 7 for ns3d.strat).  All device memory of the plan
- * is allocated here. */
+ * (4 for ns2d, 6 for ns3d, 7 for ns3d.strat).  Any grid size >= 2 per axis:
+ * power-of-two lengths run the register/LDS Stockham passes, other lengths
+ * (as FFTW takes them) mixed-radix passes (radix 8, 4, 2, 3, 5 and a direct
+ * DFT for other primes).  All device memory of the plan is allocated here,
+ * including the buffers of fsx_diag. */
 int fsx_plan_create(fsx_plan* out, int dim, int nx, int ny, int nz, double Lx,
                     double Ly, double Lz, int precision,
                     const unsigned char* where_dealiased, int nf_max);

@@ -42,9 +42,14 @@ def test_argument_errors_without_gpu():
     """Argument validation happens before any HIP call."""
     lib = _lib.get_lib()
     h = ctypes.c_void_p()
-    rc = lib.fsx_plan_create(ctypes.byref(h), 2, 48, 48, 1, 1.0, 1.0, 1.0, 8, None, 4)
+    # any size >= 2 per axis is taken (mixed-radix passes); 1 and lengths
+    # beyond the LDS tile are refused
+    rc = lib.fsx_plan_create(ctypes.byref(h), 2, 48, 1, 1, 1.0, 1.0, 1.0, 8, None, 4)
     assert rc == _lib.FSX_ERR_ARG
-    assert "power-of-two" in _lib.last_error()
+    assert ">= 2" in _lib.last_error()
+    rc = lib.fsx_plan_create(ctypes.byref(h), 3, 48, 48, 3000, 1.0, 1.0, 1.0, 8, None, 7)
+    assert rc == _lib.FSX_ERR_ARG
+    assert "too large" in _lib.last_error()
     rc = lib.fsx_plan_create(ctypes.byref(h), 4, 64, 64, 1, 1.0, 1.0, 1.0, 8, None, 4)
     assert rc == _lib.FSX_ERR_ARG
     rc = lib.fsx_plan_create(ctypes.byref(h), 2, 64, 64, 1, 1.0, 1.0, 1.0, 2, None, 4)

@@ -292,3 +292,63 @@ def test_saved_outputs(tmp_path, monkeypatch):
     E, eps, t = mem["E"], mem["epsK_tot"], mem["t"]
     rate = np.diff(E) / np.diff(t)
     assert np.allclose(rate, -0.5 * (eps[1:] + eps[:-1]), rtol=2e-3)
+
+
+def test_forced_saved_means_see_the_step_dt_and_forcing():
+    """Forced ns2d (tcrandom, forcing_rate 0.4) with CFL and a spatial-means
+    save at every step: every record after t = 0 has PZ1 + PZ2 equal to the
+    forcing rate to 1e-10, i.e. the outputs at t see the dt and the forcing
+    normalised for the step that starts at t (fluidsim/base/time_stepping/
+    base.py:208-244 orders CFL, forcing.compute, outputs, step).  The dt
+    printed and saved at t is the dt of that step, and the steps match a run
+    without saves bit for bit."""
+    from fluidsim_amd.solvers.ns2d.solver import Simul
+
+    n = 64
+
+    def make(save):
+        params = Simul.create_default_params()
+        params.oper.nx = params.oper.ny = n
+        _common(params)
+        params.nu_2 = 1e-3
+        params.forcing.enable = True
+        params.forcing.type = "tcrandom"
+        params.forcing.nkmin_forcing = 3
+        params.forcing.nkmax_forcing = 5
+        params.forcing.forcing_rate = 0.4
+        params.time_stepping.USE_CFL = True
+        params.time_stepping.cfl_coef = 0.5
+        params.time_stepping.USE_T_END = False
+        params.time_stepping.it_end = 8
+        if save:
+            params.output.periods_save.spatial_means = 1e-9
+        np.random.seed(11)
+        sim = Simul(params)
+        sim.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(n, n, seed=4, kmax=10))
+        return sim
+
+    sim = make(True)
+    dts_seen, dts_used = [], []
+    out_step = sim.output.one_time_step
+    orig = sim.time_stepping.one_time_step_computation
+
+    def out_record():
+        dts_seen.append(sim.time_stepping.deltat)
+        out_step()
+
+    def step_record():
+        dts_used.append(sim.time_stepping.deltat)
+        orig()
+
+    sim.output.one_time_step = out_record
+    sim.time_stepping.one_time_step_computation = step_record
+    sim.time_stepping.start()
+    m = sim.output.load_spatial_means()
+    assert len(m["t"]) >= 7
+    for i in range(1, len(m["t"])):
+        assert m["PZ1"][i] + m["PZ2"][i] == pytest.approx(0.4, rel=1e-10), i
+    np.testing.assert_array_equal(dts_seen, dts_used)
+    ref = make(False)
+    ref.time_stepping.start()
+    np.testing.assert_array_equal(sim.state.get_var_numpy("rot_fft"),
+                                  ref.state.get_var_numpy("rot_fft"))
