@@ -1,0 +1,123 @@
+"""Asserted properties at the bench sizes (512^3 and 1024^3), where the
+FP64 oracle is too slow to run: the tile choices timed by bench.py (the
+1024-thread multi-transform z-tiles, the register row pass with its
+spill-prone strat variant) are checked here through size-independent
+properties.
+
+* Energy conservation of the nonlinear term with 2/3 dealiasing
+  (fluidsim/solvers/ns3d/test_solver.py:80-96; with the potential energy
+  for ns3d.strat, fluidsim/solvers/ns3d/strat/test_solver.py:35-58):
+  |sum_k w Re(v* . N)| / sum_k w |v| |N| <= 1e-13 (FP64), <= 2e-6 (FP32).
+* FP32 against FP64 at 512^3 on the same seeded fields and dt, two RK4
+  steps: relative L2 <= 2e-6 per step.
+"""
+
+import functools
+
+import numpy as np
+import pytest
+
+from fluidsim_amd.synthetic import ns3d_spectrum_vv_fft, nu8_for_kmax_eta
+
+pytestmark = pytest.mark.gpu
+
+
+@functools.lru_cache(maxsize=1)
+def _vv512():
+    return ns3d_spectrum_vv_fft(512, seed=1)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _sim(n, precision, strat=False, vv=None, device_init=False):
+    if strat:
+        from fluidsim_amd.solvers.ns3d.strat.solver import Simul
+    else:
+        from fluidsim_amd.solvers.ns3d.solver import Simul
+    params = Simul.create_default_params()
+    params.oper.nx = params.oper.ny = params.oper.nz = n
+    params.oper.precision = precision
+    params.nu_8 = nu8_for_kmax_eta(n)
+    if strat:
+        params.N = 4.0
+    params.time_stepping.USE_CFL = False
+    params.time_stepping.deltat0 = 2e-3 * 512 / n
+    params.init_fields.type = "in_script"
+    params.output.HAS_TO_SAVE = False
+    params.output.periods_print.print_stdout = 0
+    sim = Simul(params)
+    if device_init:
+        from fluidsim_amd.synthetic import ns3d_spectrum_vv_device
+
+        vv = ns3d_spectrum_vv_device(sim.oper, seed=1)
+    if strat:
+        b = 0.5 * vv[1]
+        sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2], b_fft=b)
+    else:
+        sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+    return sim
+
+
+def _energy_transfer_ratio(sim, strat):
+    """|sum w Re(conj(S) . N)| / sum w |S| |N| over the state variables (for
+    strat the b term is weighted by 1/N^2: E_A = |b|^2 / (2 N^2))."""
+    torch = _torch()
+    S = sim.state.state_spect.data
+    N = sim.tendencies_nonlin(sim.state.state_spect).data
+    oper = sim.oper
+    num = den = 0.0
+    for i in range(S.shape[0]):
+        w = 1.0 / sim.params.N**2 if (strat and i == 3) else 1.0
+        a, b = S[i], N[i]
+        num += w * oper.sum_wavenumbers((a.conj() * b).real)
+        den += w * oper.sum_wavenumbers(a.abs() * b.abs())
+        del a, b
+    torch.cuda.synchronize()
+    return abs(num) / den
+
+
+@pytest.mark.parametrize("strat", [False, True])
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_energy_conservation_512(precision, strat):
+    n = 512
+    print(f"case energy transfer {n}^3 {precision} strat={strat}", flush=True)
+    sim = _sim(n, precision, strat, vv=_vv512())
+    r = _energy_transfer_ratio(sim, strat)
+    print(f"{n}^3 {precision} strat={strat}: energy transfer ratio = {r:.3e}")
+    assert r <= (1e-13 if precision == "float64" else 2e-6)
+
+
+@pytest.mark.parametrize("strat", [False, True])
+def test_energy_conservation_1024_fp32(strat):
+    n = 1024
+    print(f"case energy transfer {n}^3 float32 strat={strat}", flush=True)
+    sim = _sim(n, "float32", strat, device_init=True)
+    r = _energy_transfer_ratio(sim, strat)
+    print(f"{n}^3 float32 strat={strat}: energy transfer ratio = {r:.3e}")
+    assert r <= 2e-6
+    # two steps stay finite (the NaN check of every step raises otherwise)
+    for _ in range(2):
+        sim.time_stepping.one_time_step_computation()
+
+
+def test_fp32_vs_fp64_512_two_steps():
+    torch = _torch()
+    n = 512
+    print(f"case FP32 vs FP64 {n}^3", flush=True)
+    s64 = _sim(n, "float64", vv=_vv512())
+    s32 = _sim(n, "float32", vv=_vv512())
+    errs = []
+    for step in range(2):
+        for s in (s64, s32):
+            s.time_stepping.one_time_step_computation()
+        a, r = s32.state.state_spect.data, s64.state.state_spect.data
+        num = sum(float(torch.linalg.vector_norm(a[f].to(r.dtype) - r[f]).item()) ** 2
+                  for f in range(3))
+        den = sum(float(torch.linalg.vector_norm(r[f]).item()) ** 2 for f in range(3))
+        errs.append((num / den) ** 0.5)
+    print(f"FP32 vs FP64 {n}^3 after 1, 2 steps: {errs}")
+    assert errs[0] <= 2e-6 and errs[1] <= 4e-6
