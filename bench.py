@@ -959,6 +959,84 @@ def bench_closure(args, torch, _lib):
     print(json.dumps(result), flush=True)
 
 
+def bench_cnn_train(args, torch, _lib):
+    """Training of the closure network on the device: Adam steps of the
+    6-layer 5x5 32-channel network at batch B on H x H images (synthetic
+    inputs and targets), and the weight gradient of one hidden 32 -> 32
+    layer (fsx_cnn_conv_wgrad: the MFMA kernel, the reduction of its
+    partials and the bias gradient) timed alone with HIP events on the
+    stream it runs on."""
+    import ctypes
+
+    from fluidsim_amd import closure as CL
+
+    B, H = args.cnn_batch, args.cnn_size
+    npix = B * H * H
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn((B, H, H, 1), generator=g).cuda()
+    tgt = torch.randn((B, H, H), generator=g).cuda()
+    net = CL.CNNClosure(n_in=1, nlayers=6, seed=0)
+    for _ in range(args.warmup):
+        net.train_step(x, tgt, want_loss=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.train_step(x, tgt, want_loss=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # one hidden layer's weight gradient, alone
+    a = torch.randint(-16000, 16000, (B, H, H, 32), dtype=torch.int16, generator=g).cuda()
+    d = torch.randint(-16000, 16000, (B, H, H, 32), dtype=torch.int16, generator=g).cuda()
+    nb = ctypes.c_longlong(0)
+    _lib.call("fsx_cnn_wgrad_workspace", B, H, 32, ctypes.byref(nb))
+    work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    dw = torch.empty((32, 25, 32), dtype=torch.float32, device="cuda")
+    db = torch.empty(32, dtype=torch.float32, device="cuda")
+    s = _lib.stream_ptr()
+
+    def wgrad():
+        _lib.call("fsx_cnn_conv_wgrad", _lib.dptr(a), 32, _lib.dptr(d), 32, B, H, H,
+                  _lib.dptr(dw), _lib.dptr(db), _lib.dptr(work), nb.value, s)
+
+    for _ in range(3):
+        wgrad()
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        wgrad()
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    flops = 2.0 * npix * 32 * 25 * 32
+    tflops = flops / (us * 1e-6) / 1e12
+    result = {
+        "metric": "closure network training steps/s",
+        "value": args.steps / elapsed,
+        "unit": "Adam steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 x bf16 -> f32",
+        "data": "synthetic",
+        "config": {"workload": f"6-layer 5x5 32-channel GELU network, batch {B} of "
+                               f"{H}x{H}, MSE loss, Adam", "parallelism": "single GPU"},
+        "roofline": {
+            "bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": tflops / 2500.0, "traffic": None,
+            "kernel": "fsx_cnn_conv_wgrad, hidden 32 -> 32 layer (k_conv_wgrad<32, 32> "
+                      "+ reduction of the partials), HIP events",
+            "us_per_call": us,
+            "flops_per_call": flops,
+        },
+    }
+    print(json.dumps(result), flush=True)
+
+
 def cpu_baseline_closure(net, nd, nl, nu, reps=2):
     """FP64 oracle on the host cores: one LES RK4 step with the network at
     nl^2 (the ns2d step plus 4 network forwards, numpy) and one DNS RK4 step
@@ -1043,6 +1121,10 @@ def main():
     ap.add_argument("--closure-train-steps", type=int, default=2000)
     ap.add_argument("--closure-eval-t", type=float, default=4.0)
     ap.add_argument("--closure-sample-dt", type=float, default=0.2)
+    ap.add_argument("--cnn-train", action="store_true",
+                    help="time the closure network's training step and weight gradient")
+    ap.add_argument("--cnn-batch", type=int, default=32)
+    ap.add_argument("--cnn-size", type=int, default=128)
     ap.add_argument("--ns2d", action="store_true",
                     help="time the forced ns2d DNS of the closure workload (use --n 2048)")
     ap.add_argument("--strat", action="store_true",
@@ -1080,6 +1162,8 @@ def main():
         return bench_ensemble(args, torch, _lib, copy_gbps)
     if args.closure:
         return bench_closure(args, torch, _lib)
+    if args.cnn_train:
+        return bench_cnn_train(args, torch, _lib)
     if args.strat:
         return bench_strat(args, torch, _lib, copy_gbps)
     if args.ns2d:

@@ -106,6 +106,41 @@ int cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
   return 0;
 }
 
+// Weight and bias gradients into dw / dbias through the partials buffer
+// part (wgrad_groups(B, H) x (32 x 25 ci + 32) floats); arguments checked
+// by the caller.
+int cnn_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W, float* dw,
+              float* dbias, float* part, hipStream_t s) {
+  const size_t lds = conv_wgrad_lds(ci, W);
+  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "image too wide for the weight-gradient tile");
+  const int groups = wgrad_groups(B, H);
+  const auto* xi = reinterpret_cast<const uint16_t*>(x);
+  const auto* di = reinterpret_cast<const uint16_t*>(dy);
+  int rc;
+#define FSX_WG(CI_, CO_)                                                              \
+  if ((rc = set_lds(k_conv_wgrad<CI_, CO_>, lds))) return rc;                          \
+  hipLaunchKernelGGL((k_conv_wgrad<CI_, CO_>), dim3(groups), dim3(kWgThreads), lds, s, xi, di, \
+                     part, B, H, W);
+  if (ci == 8 && co_in == 8) {
+    FSX_WG(8, 8)
+  } else if (ci == 8) {
+    FSX_WG(8, 32)
+  } else if (co_in == 8) {
+    FSX_WG(32, 8)
+  } else {
+    FSX_WG(32, 32)
+  }
+#undef FSX_WG
+  FSX_HIP(hipGetLastError());
+  const int n = 32 * kCnnTaps * ci;
+  hipLaunchKernelGGL(k_cnn_reduce, dim3((n + 63) / 64), dim3(256), 0, s, part, groups, n, dw);
+  FSX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cnn_reduce, dim3(1), dim3(256), 0, s, part + (size_t)groups * n, groups,
+                     32, dbias);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
 // workspace of fsx_ns2d_closure_term: vorticity and network output (FP32),
 // the network's activations
 long long closure_work(const fsx_plan_s* p) {
@@ -157,10 +192,9 @@ int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void*
 
 int fsx_cnn_wgrad_workspace(int B, int H, int ci, long long* bytes) {
   if (!bytes || B < 1 || H < kCnnRows || !cnn_channels_ok(ci)) return fail(FSX_ERR_ARG, "bad argument");
-  const long long nblk = (long long)B * (H / kCnnRows);
-  const long long groups = std::min<long long>(nblk, 256);
+  const long long groups = wgrad_groups(B, H);
   // weight partials, then bias partials
-  *bytes = groups * 32LL * kCnnTaps * ci * 4 + 1024LL * 32 * 4;
+  *bytes = groups * 32LL * kCnnTaps * ci * 4 + groups * 32LL * 4;
   return 0;
 }
 
@@ -175,40 +209,8 @@ int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, 
   long long need = 0;
   fsx_cnn_wgrad_workspace(B, H, ci, &need);
   if (work_bytes < need) return fail(FSX_ERR_ARG, "workspace too small");
-  const size_t lds = conv_wgrad_lds(ci, W);
-  if (lds > kLdsMax) return fail(FSX_ERR_ARG, "image too wide for the weight-gradient tile");
-  hipStream_t s = (hipStream_t)stream;
-  const long long nblk = (long long)B * (H / kCnnRows);
-  const int groups = (int)std::min<long long>(nblk, 256);
-  float* part = reinterpret_cast<float*>(work);
-  const auto* xi = reinterpret_cast<const uint16_t*>(x);
-  const auto* di = reinterpret_cast<const uint16_t*>(dy);
-  int rc;
-#define FSX_WG(CI_, CO_)                                                              \
-  if ((rc = set_lds(k_conv_wgrad<CI_, CO_>, lds))) return rc;                          \
-  hipLaunchKernelGGL((k_conv_wgrad<CI_, CO_>), dim3(groups), dim3(256), lds, s, xi, di, \
-                     part, B, H, W);
-  if (ci == 8 && co_in == 8) {
-    FSX_WG(8, 8)
-  } else if (ci == 8) {
-    FSX_WG(8, 32)
-  } else if (co_in == 8) {
-    FSX_WG(32, 8)
-  } else {
-    FSX_WG(32, 32)
-  }
-#undef FSX_WG
-  FSX_HIP(hipGetLastError());
-  const int n = 32 * kCnnTaps * ci;
-  hipLaunchKernelGGL(k_cnn_reduce, dim3((n + 63) / 64), dim3(256), 0, s, part, groups, n, dw);
-  FSX_HIP(hipGetLastError());
-  float* bpart = part + (size_t)groups * n;
-  const long long npix = (long long)B * H * W;
-  hipLaunchKernelGGL(k_cnn_bias_grad, dim3(1024), dim3(256), 0, s, di, npix, co_in, bpart);
-  FSX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cnn_reduce, dim3(1), dim3(256), 0, s, bpart, 1024, 32, dbias);
-  FSX_HIP(hipGetLastError());
-  return 0;
+  return cnn_wgrad(x, ci, dy, co_in, B, H, W, dw, dbias, reinterpret_cast<float*>(work),
+                   (hipStream_t)stream);
 }
 
 int fsx_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,

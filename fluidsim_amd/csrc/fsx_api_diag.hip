@@ -185,23 +185,29 @@ int ns2d_budget(fsx_plan_s* p, const void* rot, double beta, void* wk, void* wx,
 }  // namespace
 
 // Buffers of fsx_diag, allocated with the plan: sized for the largest block
-// records and output over the diagnostics kinds of the plan's dimension with
-// every histogram (fewer histograms need less).  A grid too large for the
-// LDS histograms gets none, and fsx_diag refuses it.
+// records and output over the diagnostics kinds of the plan's dimension and
+// every histogram set the grid admits (a large grid may admit only some of
+// the histograms at once; sets it refuses are skipped).  A plan whose grid
+// admits none gets no buffers, and fsx_diag refuses it.
 int fsx_diag_alloc(fsx_plan_s* p) {
   size_t part = 0, out = 0;
+  const std::string err = fsx_g_err;  // refused sets are not errors here
   for (int kind = 0; kind < DIAG_NKINDS; ++kind) {
     const bool kind2d = kind == DIAG_NS2D_SPECTRA || kind == DIAG_NS2D_BUDGET ||
                         kind == DIAG_NS2D_MEANS;
     if (kind2d != (p->dim == 2)) continue;
     const bool means = kind >= DIAG_NS2D_MEANS;
-    const int flags = means ? 0 : (p->dim == 3 ? 31 : 31 & ~(DIAG_KZ | DIAG_KZKH));
-    size_t pa = 0, ou = 0;
-    int qstep = 1;
-    if (diag_sizes(p, kind, flags, pa, ou, qstep)) return 0;
-    part = std::max(part, pa);
-    out = std::max(out, ou);
+    const int all = means ? 0 : (p->dim == 3 ? 31 : 31 & ~(DIAG_KZ | DIAG_KZKH));
+    for (int flags = 0; flags <= all; ++flags) {
+      if ((flags & all) != flags) continue;
+      size_t pa = 0, ou = 0;
+      int qstep = 1;
+      if (diag_sizes(p, kind, flags, pa, ou, qstep)) continue;
+      part = std::max(part, pa);
+      out = std::max(out, ou);
+    }
   }
+  fsx_g_err = err;
   if (!part || !out) return 0;
   FSX_HIP(hipMalloc((void**)&p->d_dpart, part * sizeof(double)));
   p->dpart_cap = part;

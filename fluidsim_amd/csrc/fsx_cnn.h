@@ -2,8 +2,8 @@
 //
 // Network: 5x5 periodic convolutions (the domain is periodic), 32 channels,
 // GELU between layers; BF16 weights and activations, FP32 accumulation on
-// MFMA (v_mfma_f32_32x32x16_bf16 in the convolutions, the 32x32x8 form in
-// the weight gradients), FP32 pre-activations, FP32 master weights with Adam.
+// MFMA (v_mfma_f32_32x32x16_bf16 in the convolutions and the weight
+// gradients), FP32 pre-activations, FP32 master weights with Adam.
 //
 // Layouts (channels last):
 //   activations  x[B][H][W][C]  BF16, C = 8 or 32 (channel-padded with 0)
@@ -253,88 +253,256 @@ static __global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad
   }
 }
 
-// LDS bytes of k_conv_wgrad: dy rows [rows][32][W + 5] + x patch
-// [rows + 4][CI][W + 9] (BF16; odd pitches spread the 16-bit reads of
-// neighbouring lanes over the banks).
+// Weight gradient dw[o][t][c] = sum_p dy[p][o] x[p + d(t)][c] and bias
+// gradient db[o] = sum_p dy[p][o], as a GEMM over the pixels p (the MFMA K
+// dimension) on v_mfma_f32_32x32x16_bf16: M = 32 output channels o,
+// N = 32 input channels c of one tap.
+//
+// Row blocks of kWgRows output rows.  Staging transposes to pixels-fast in
+// LDS: dy as [kWgRows][32 o][DP], the x patch (rows y0 - 2 .. y0 + kWgRows
+// + 1, patch column j = image column j - 2, periodic) as
+// [kWgRows + 4][32 c][XP].  Channel row c starts wg_pad(c) elements into
+// its slot and each patch row is 32 elements longer than its 32 channel
+// rows, so the transposed 8-byte writes of the lanes of one pixel run (4
+// channel vectors) spread over the banks; channel rows beyond CI / CO_IN
+// are zero (written once), so the MMA loop reads every lane unmasked.
+// A thread loads 8 pixels x 8 channels with 16-byte loads (two halves of 4
+// pixels; lanes of one pixel run are consecutive, 16 cache lines per wave
+// instruction), transposes each 4 x 8 block of BF16 in registers and writes
+// 8 rows of 8 bytes.  Both pitches are odd multiples of 8 elements, so the
+// 16-byte reads of 8 consecutive lanes (channels) fall in distinct banks.
+//
+// Wave w (5 waves) owns the taps of row dy = w.  Per 16 pixels it reads the
+// dy operand once and two aligned 16-byte x runs per lane, one chunk ahead
+// of the MFMAs; the operands of taps dx = 0..4 (x columns shifted by dx) are
+// the two runs shifted by dx elements (v_alignbit), so all LDS reads stay
+// 16-byte aligned.  Wave 0 also multiplies dy by a column of ones: the bias
+// gradient.
+//
+// Persistent: workgroup g (of G) takes row blocks g, g + G, ... of its
+// XCD's contiguous share (neighbouring row blocks share halo rows in that
+// XCD's L2) and writes its partials part[g][32][25 CI] and, after all G
+// weight partials, part[G 32 25 CI + 32 g + o]; k_cnn_reduce sums them in a
+// fixed order (deterministic).
+constexpr int kWgRows = 2;      // output rows per row block
+constexpr int kWgThreads = 320; // 5 waves, one per tap row
+constexpr int kWgGroups = 512;  // persistent workgroups (2 per CU)
+
+__host__ __device__ inline int wg_dp(int W) { return odd8(W); }
+__host__ __device__ inline int wg_xp(int W) { return odd8(W + 8); }
+__host__ __device__ inline int wg_pad(int c) { return 8 * ((c >> 3) & 3); }
+// LDS elements of one dy row / one x patch row (32 channel rows + pads)
+__host__ __device__ inline int wg_drow(int W) { return 32 * wg_dp(W) + 32; }
+__host__ __device__ inline int wg_xrow(int W) { return 32 * wg_xp(W) + 32; }
+// LDS bytes of k_conv_wgrad
 __host__ __device__ inline size_t conv_wgrad_lds(int ci, int W) {
-  return (size_t)2 * ((size_t)kCnnRows * 32 * (W + 5) + (size_t)(kCnnRows + 4) * ci * (W + 9));
+  (void)ci;
+  return (size_t)2 * ((size_t)kWgRows * wg_drow(W) + (size_t)(kWgRows + 4) * wg_xrow(W));
+}
+// Workgroups of k_conv_wgrad for B images of H rows.
+inline int wgrad_groups(long long B, int H) {
+  const long long nblk = B * (H / kWgRows);
+  if (nblk >= kWgGroups) return kWgGroups;
+  return (int)nblk;
 }
 
-// Weight gradient, persistent: workgroup g takes the row blocks
-// g, g + G, ... of all images, accumulates dw in MFMA registers and writes
-// its partial part[g][32][25 * CI]; k_cnn_reduce sums the partials.
-// Each wave owns the N tiles (32 columns of (tap, ci)) wid, wid + 4, ...
+// Transposes 4 pixels x 8 channels of BF16 (v[j]: channels of pixel j, two
+// per dword, low half first) into t[c]: the 4 pixels of channel c.
+__device__ __forceinline__ void bf16_transpose4x8(const u32x4 (&v)[4], uint2 (&t)[8]) {
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    t[2 * d].x = (v[0][d] & 0xffffu) | (v[1][d] << 16);
+    t[2 * d].y = (v[2][d] & 0xffffu) | (v[3][d] << 16);
+    t[2 * d + 1].x = (v[0][d] >> 16) | (v[1][d] & 0xffff0000u);
+    t[2 * d + 1].y = (v[2][d] >> 16) | (v[3][d] & 0xffff0000u);
+  }
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// MFMAs of one 16-pixel chunk: dy operand a, x runs x0 (columns px .. px + 7
+// of the patch) and x1 (px + 8 .. px + 15); tap dx reads px + dx .. + 7.
+template <bool BIAS>
+__device__ __forceinline__ void wg_chunk(const u32x4& a, const u32x4& x0, const u32x4& x1,
+                                         const u32x4& ones, f32x16 (&acc)[5], f32x16& accb) {
+  const bf16x8 A = as_bf16x8(a);
+  const u32x4 b1 = {__builtin_amdgcn_alignbit(x0[1], x0[0], 16),
+                    __builtin_amdgcn_alignbit(x0[2], x0[1], 16),
+                    __builtin_amdgcn_alignbit(x0[3], x0[2], 16),
+                    __builtin_amdgcn_alignbit(x1[0], x0[3], 16)};
+  const u32x4 b2 = {x0[1], x0[2], x0[3], x1[0]};
+  const u32x4 b3 = {__builtin_amdgcn_alignbit(x0[2], x0[1], 16),
+                    __builtin_amdgcn_alignbit(x0[3], x0[2], 16),
+                    __builtin_amdgcn_alignbit(x1[0], x0[3], 16),
+                    __builtin_amdgcn_alignbit(x1[1], x1[0], 16)};
+  const u32x4 b4 = {x0[2], x0[3], x1[0], x1[1]};
+  acc[0] = mfma16_bf16(A, as_bf16x8(x0), acc[0]);
+  acc[1] = mfma16_bf16(A, as_bf16x8(b1), acc[1]);
+  acc[2] = mfma16_bf16(A, as_bf16x8(b2), acc[2]);
+  acc[3] = mfma16_bf16(A, as_bf16x8(b3), acc[3]);
+  acc[4] = mfma16_bf16(A, as_bf16x8(b4), acc[4]);
+  if constexpr (BIAS) accb = mfma16_bf16(A, as_bf16x8(ones), accb);
+}
+
+// MFMA loop over the staged row block: full 16-pixel chunks with the loads
+// of the next chunk issued before the MFMAs of the current one, then, for
+// W % 16 == 8, the last 8 pixels (upper half-wave supplies zeros).
+template <bool BIAS>
+__device__ __forceinline__ void wg_rows(const uint16_t* pa0, const uint16_t* px_0, int W,
+                                        int DROW, int XROW, int g, f32x16 (&acc)[5],
+                                        f32x16& accb) {
+  const u32x4 ones = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};
+  const int wfull = W & ~15;
+#pragma unroll
+  for (int r = 0; r < kWgRows; ++r) {
+    const uint16_t* pa = pa0 + r * DROW;
+    const uint16_t* pxr = px_0 + r * XROW;
+    u32x4 a = *reinterpret_cast<const u32x4*>(pa);
+    u32x4 x0 = *reinterpret_cast<const u32x4*>(pxr);
+    u32x4 x1 = *reinterpret_cast<const u32x4*>(pxr + 8);
+    for (int px0 = 16; px0 < wfull; px0 += 16) {
+      const u32x4 an = *reinterpret_cast<const u32x4*>(pa + px0);
+      const u32x4 x0n = *reinterpret_cast<const u32x4*>(pxr + px0);
+      const u32x4 x1n = *reinterpret_cast<const u32x4*>(pxr + px0 + 8);
+      wg_chunk<BIAS>(a, x0, x1, ones, acc, accb);
+      a = an;
+      x0 = x0n;
+      x1 = x1n;
+    }
+    wg_chunk<BIAS>(a, x0, x1, ones, acc, accb);
+    if (W & 8) {
+      const u32x4 z = {0u, 0u, 0u, 0u};
+      u32x4 at = z, x0t = z, x1t = z;
+      if (g == 0) {
+        at = *reinterpret_cast<const u32x4*>(pa + wfull);
+        x0t = *reinterpret_cast<const u32x4*>(pxr + wfull);
+        x1t = *reinterpret_cast<const u32x4*>(pxr + wfull + 8);
+      }
+      wg_chunk<BIAS>(at, x0t, x1t, g == 0 ? ones : z, acc, accb);
+    }
+  }
+}
+
 template <int CI, int CO_IN>
-static __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ x,
-                                                    const uint16_t* __restrict__ dy,
-                                                    float* __restrict__ part, int B, int H,
-                                                    int W) {
+static __global__ __launch_bounds__(kWgThreads) __attribute__((amdgpu_waves_per_eu(3)))
+void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                  float* __restrict__ part, int B, int H, int W) {
   constexpr int N = kCnnTaps * CI;
-  constexpr int NT = (N + 31) / 32;
-  constexpr int TPW = (NT + 3) / 4;  // tiles per wave
+  constexpr int NVI = CI / 8, NVO = CO_IN / 8;  // 16-byte vectors per pixel
   extern __shared__ __align__(16) uint16_t cnn_sm[];
-  const int DP = W + 5, XP = W + 9;
-  uint16_t* sd = cnn_sm;                            // [rows][32][DP]
-  uint16_t* sx = cnn_sm + kCnnRows * 32 * DP;       // [rows + 4][CI][XP]
+  const int DP = wg_dp(W), XP = wg_xp(W);
+  const int DROW = wg_drow(W), XROW = wg_xrow(W);
+  uint16_t* sd = cnn_sm;                    // [kWgRows][DROW]
+  uint16_t* sx = cnn_sm + kWgRows * DROW;   // [kWgRows + 4][XROW]
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int m = lane & 31, g = lane >> 5;
-  f32x16 acc[TPW];
-#pragma unroll
-  for (int q = 0; q < TPW; ++q) acc[q] = f32x16{};
-  const int nyb = H / kCnnRows;
-  const int nblk = B * nyb;
-  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-    const int b = blk / nyb, y0 = (blk % nyb) * kCnnRows;
-    __syncthreads();
-    // dy rows, transposed to [r][o][x]; channels beyond CO_IN are zero
-    for (int t = threadIdx.x; t < kCnnRows * W * 32; t += blockDim.x) {
-      const int o = t % 32, rest = t / 32, px = rest % W, r = rest / W;
-      uint16_t v = 0;
-      if (o < CO_IN) v = dy[(((size_t)b * H + y0 + r) * W + px) * CO_IN + o];
-      sd[(r * 32 + o) * DP + px] = v;
-    }
-    // x rows y0 - 2 .. y0 + rows + 1, columns -2 .. W + 1, transposed
-    for (int t = threadIdx.x; t < (kCnnRows + 4) * (W + 4) * CI; t += blockDim.x) {
-      const int c = t % CI, rest = t / CI, col = rest % (W + 4), r = rest / (W + 4);
-      const int gy = (y0 - 2 + r + H) % H, gx = (col - 2 + W) % W;
-      sx[(r * CI + c) * XP + col] = x[(((size_t)b * H + gy) * W + gx) * CI + c];
-    }
-    __syncthreads();
-    for (int k0 = 0; k0 < kCnnRows * W; k0 += 8) {
-      // 8 consecutive pixels of one row (W is a multiple of 8)
-      const int r = k0 / W, px0 = k0 % W + 4 * g;
-      bf16x4 a;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = (short)sd[(r * 32 + m) * DP + px0 + j];
-#pragma unroll
-      for (int q = 0; q < TPW; ++q) {
-        const int tile = wid + 4 * q;
-        if (tile >= NT) break;
-        const int n = tile * 32 + m;
-        bf16x4 bb = {0, 0, 0, 0};
-        if (n < N) {
-          const int tap = n / CI, c = n % CI;
-          const int tdy = tap / 5, tdx = tap % 5;
-          const uint16_t* src = &sx[((r + tdy) * CI + c) * XP + px0 + tdx];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) bb[j] = (short)src[j];
+  // channel rows that staging never writes: zero once
+  if (CI < 32 || CO_IN < 32) {
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (int r = 0; r < kWgRows + 4; ++r) {
+      if (r < kWgRows)
+        for (int i = threadIdx.x; i < (32 - CO_IN) * (DP / 8); i += kWgThreads) {
+          const int o = CO_IN + i / (DP / 8);
+          *reinterpret_cast<u32x4*>(sd + r * DROW + o * DP + wg_pad(o) + 8 * (i % (DP / 8))) = z;
         }
-        acc[q] = mfma_bf16(a, bb, acc[q]);
+      for (int i = threadIdx.x; i < (32 - CI) * (XP / 8); i += kWgThreads) {
+        const int c = CI + i / (XP / 8);
+        *reinterpret_cast<u32x4*>(sx + r * XROW + c * XP + wg_pad(c) + 8 * (i % (XP / 8))) = z;
       }
     }
   }
-  float* dst = part + (size_t)blockIdx.x * 32 * N;
+  f32x16 acc[5], accb = {};
 #pragma unroll
-  for (int q = 0; q < TPW; ++q) {
-    const int tile = wid + 4 * q;
-    if (tile >= NT) break;
-    const int n = tile * 32 + m;
-    if (n >= N) continue;
+  for (int q = 0; q < 5; ++q) acc[q] = f32x16{};
+  const int nyb = H / kWgRows;
+  const int nblk = B * nyb;
+  const int G = gridDim.x;
+  // XCD-contiguous shares of the row blocks (blocks go round-robin to the
+  // 8 XCDs); plain round-robin when G is not a multiple of 8
+  int first, stride, last;
+  if (G % 8 == 0) {
+    const int gx = G / 8, xcd = blockIdx.x % 8, i = blockIdx.x / 8;
+    const int share = (nblk + 7) / 8;
+    first = xcd * share + i;
+    stride = gx;
+    last = min(nblk, (xcd + 1) * share);
+  } else {
+    first = blockIdx.x;
+    stride = G;
+    last = nblk;
+  }
+  const int ncg = W / 8 + 1;  // 8-column groups of the x patch
+  // this lane's operand rows (pixels-fast): dy channel m, x channel m
+  const uint16_t* pa0 = sd + m * DP + wg_pad(m) + 8 * g;
+  const uint16_t* px_0 = sx + wid * XROW + m * XP + wg_pad(m) + 8 * g;
+  for (int blk = first; blk < last; blk += stride) {
+    const int b = blk / nyb, y0 = (blk - b * nyb) * kWgRows;
+    __syncthreads();
+    // x patch: item = (patch row, column group, channel vector), channel
+    // vector fastest
+    for (int it = threadIdx.x; it < (kWgRows + 4) * NVI * ncg; it += kWgThreads) {
+      const int q = it % NVI, rk = it / NVI;
+      const int k = rk % ncg, pr = rk / ncg;
+      int gy = y0 - 2 + pr;
+      gy += gy < 0 ? H : (gy >= H ? -H : 0);
+      const uint16_t* row = x + ((size_t)b * H + gy) * W * CI + 8 * q;
+      uint16_t* dst = sx + pr * XROW + 8 * q * XP + wg_pad(8 * q) + 8 * k;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int o = 8 * (i >> 2) + 4 * g + (i & 3);
-      dst[(size_t)o * N + n] = acc[q][i];
+      for (int h = 0; h < 2; ++h) {
+        u32x4 v[4];
+        uint2 t[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          int gx = 8 * k - 2 + 4 * h + j;
+          gx += gx < 0 ? W : (gx >= W ? -W : 0);
+          v[j] = *reinterpret_cast<const u32x4*>(row + (size_t)gx * CI);
+        }
+        bf16_transpose4x8(v, t);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) *reinterpret_cast<uint2*>(dst + c * XP + 4 * h) = t[c];
+      }
     }
+    // dy rows: item = (row, column group, channel vector)
+    for (int it = threadIdx.x; it < kWgRows * NVO * (W / 8); it += kWgThreads) {
+      const int q = it % NVO, rk = it / NVO;
+      const int k = rk % (W / 8), r = rk / (W / 8);
+      const uint16_t* src = dy + (((size_t)b * H + y0 + r) * W + 8 * k) * CO_IN + 8 * q;
+      uint16_t* dst = sd + r * DROW + 8 * q * DP + wg_pad(8 * q) + 8 * k;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        u32x4 v[4];
+        uint2 t[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[j] = *reinterpret_cast<const u32x4*>(src + (4 * h + j) * CO_IN);
+        bf16_transpose4x8(v, t);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) *reinterpret_cast<uint2*>(dst + c * DP + 4 * h) = t[c];
+      }
+    }
+    __syncthreads();
+    if (wid == 0)
+      wg_rows<true>(pa0, px_0, W, DROW, XROW, g, acc, accb);
+    else
+      wg_rows<false>(pa0, px_0, W, DROW, XROW, g, acc, accb);
+  }
+  // partials: n = (5 w + dx) CI + c, lanes m < CI
+  float* dst = part + (size_t)blockIdx.x * 32 * N;
+  if (m < CI) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const int n = (5 * wid + q) * CI + m;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int o = 8 * (i >> 2) + 4 * g + (i & 3);
+        dst[(size_t)o * N + n] = acc[q][i];
+      }
+    }
+  }
+  if (wid == 0 && m == 0) {
+    float* bp = part + (size_t)G * 32 * N + (size_t)blockIdx.x * 32;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) bp[8 * (i >> 2) + 4 * g + (i & 3)] = accb[i];
   }
 }
 
@@ -365,25 +533,6 @@ static __global__ __launch_bounds__(256) void k_cnn_reduce(const float* part, in
   __syncthreads();
   if (wid == 0 && i < n) out[i] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
-
-// bias gradient partials: part[blk][o] = sum over the block's pixels of dy[p][o]
-static __global__ __launch_bounds__(256) void k_cnn_bias_grad(const uint16_t* dy, long long npix,
-                                                       int co, float* part) {
-  __shared__ float red[256];
-  const int o = threadIdx.x % 32, lane8 = threadIdx.x / 32;
-  float s = 0.0f;
-  if (o < co)
-    for (long long p = blockIdx.x * 8LL + lane8; p < npix; p += gridDim.x * 8LL)
-      s += from_bf16(dy[p * co + o]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x < 32) {
-    float t = 0.0f;
-    for (int j = 0; j < 8; ++j) t += red[j * 32 + threadIdx.x];
-    part[blockIdx.x * 32 + threadIdx.x] = t;
-  }
-}
-
 
 // a[p][c] = BF16(act(y[p][c])) for c < cin, 0 up to cpad; act = GELU
 // (gelu != 0) or identity

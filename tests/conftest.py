@@ -1,3 +1,4 @@
+import gc
 import os
 import sys
 
@@ -21,3 +22,16 @@ def gpu_lib():
     from fluidsim_amd import _lib
 
     return _lib.get_lib()
+
+
+@pytest.fixture(autouse=True)
+def _release_device_memory():
+    """After each test: collect the simulations left in reference cycles
+    (their plans hold device workspaces outside torch's allocator) and hand
+    torch's cached blocks back, so a later test can allocate a large plan."""
+    yield
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()

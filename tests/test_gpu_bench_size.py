@@ -102,6 +102,24 @@ def test_energy_conservation_1024_fp32(strat):
     # two steps stay finite (the NaN check of every step raises otherwise)
     for _ in range(2):
         sim.time_stepping.one_time_step_computation()
+    if not strat:
+        _check_diag_flag_sets(sim)
+
+
+def _check_diag_flag_sets(sim):
+    """Every histogram set of the ns3d spectra at 1024^3 runs with the
+    buffers allocated at plan creation (fewer histograms per launch pack more
+    quantities into one launch, so a subset can need larger block records
+    than the full set) and gives the histograms of the full set."""
+    from fluidsim_amd.base import diagnostics as dg
+
+    S = sim.state.state_spect.data
+    fields = [S[i].contiguous() for i in range(3)]
+    full = dg.run(sim.oper, dg.NS3D_SPECTRA, 31, fields)
+    for flags in range(31):
+        res = dg.run(sim.oper, dg.NS3D_SPECTRA, flags, fields)
+        for key, val in res.items():
+            assert np.allclose(val, full[key], rtol=1e-12, atol=0), (flags, key)
 
 
 def test_fp32_vs_fp64_512_two_steps():
