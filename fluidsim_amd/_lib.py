@@ -113,6 +113,7 @@ SIGNATURES = {
     "fsx_cnn_conv_wgrad": [
         P, c_int, P, c_int, c_int, c_int, c_int, P, P, P, ctypes.c_longlong, P,
     ],
+    "fsx_cnn_conv_wgrad_noalloc": [P, c_int, P, c_int, c_int, c_int, c_int, P, P, P],
     "fsx_cnn_act": [P, ctypes.c_longlong, c_int, c_int, c_int, P, P],
     "fsx_cnn_act_bwd": [P, c_int, P, ctypes.c_longlong, c_int, P, P],
     "fsx_cnn_mse": [P, P, ctypes.c_longlong, c_int, P, P, P, P],

@@ -2,6 +2,8 @@
 // gradients, Adam) and of the learned-closure LES term.
 #include "fsx_internal.h"
 
+#include <mutex>
+
 namespace {
 
 bool cnn_channels_ok(int c) { return c == 8 || c == 32; }
@@ -141,6 +143,35 @@ int cnn_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, in
   return 0;
 }
 
+// Scratch of fsx_cnn_conv_wgrad_noalloc: the partials buffer of cnn_wgrad at
+// its largest (kWgGroups workgroups, 32 input channels), a device array in
+// the code object, so that the entry point allocates nothing.  Each device
+// holds its own copy; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host
+// stand-in there).
+constexpr long long kWgradScratchFloats = (long long)kWgGroups * (32LL * kCnnTaps * 32 + 32);
+static __device__ __align__(256) float scratch[kWgradScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch, resolved once per device.
+int wgrad_scratch(float** out) {
+  int dev = 0;
+  FSX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(FSX_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    FSX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(FSX_ERR_HIP, "no address for the weight-gradient scratch");
+    g_scratch_ptr[dev] = reinterpret_cast<float*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 // workspace of fsx_ns2d_closure_term: vorticity and network output (FP32),
 // the network's activations
 long long closure_work(const fsx_plan_s* p) {
@@ -211,6 +242,23 @@ int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, 
   if (work_bytes < need) return fail(FSX_ERR_ARG, "workspace too small");
   return cnn_wgrad(x, ci, dy, co_in, B, H, W, dw, dbias, reinterpret_cast<float*>(work),
                    (hipStream_t)stream);
+}
+
+int fsx_cnn_conv_wgrad_noalloc(const void* x, int ci, const void* dy, int co_in, int B, int H,
+                               int W, float* dw, float* dbias, void* stream) {
+  if (!x || !dy || !dw || !dbias || B < 1) return fail(FSX_ERR_ARG, "bad argument");
+  if (!cnn_channels_ok(ci) || !cnn_channels_ok(co_in))
+    return fail(FSX_ERR_ARG, "channels must be 8 or 32");
+  if (H < kCnnRows || H % kCnnRows || W < 8 || W % 8)
+    return fail(FSX_ERR_ARG, "H must be a multiple of 4 and W of 8");
+  long long need = 0;
+  fsx_cnn_wgrad_workspace(B, H, ci, &need);
+  if (need > kWgradScratchFloats * (long long)sizeof(float))
+    return fail(FSX_ERR_ARG, "partials exceed the weight-gradient scratch");
+  float* part = nullptr;
+  int rc = wgrad_scratch(&part);
+  if (rc) return rc;
+  return cnn_wgrad(x, ci, dy, co_in, B, H, W, dw, dbias, part, (hipStream_t)stream);
 }
 
 int fsx_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,

@@ -413,6 +413,18 @@ int fsx_cnn_wgrad_workspace(int B, int H, int ci, long long* bytes);
 int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W,
                        float* dw, float* dbias, void* work, long long work_bytes,
                        void* stream);
+/* fsx_cnn_conv_wgrad_noalloc: fsx_cnn_conv_wgrad (same kernels, same
+ *   results bit for bit) with its partials in a static device array of the
+ *   library instead of a workspace argument; it allocates nothing (no
+ *   hipMalloc, no library allocation), for callers that cannot allocate:
+ *   code under HIP graph capture, tight training loops, harnesses that
+ *   forbid allocation.  No counterpart in the reference.  The array holds
+ *   the largest partials buffer (512 x (32 x 25 x 32 + 32) floats, 52.5 MB
+ *   per device, held until the process ends); sizes beyond it are refused.
+ *   Every call on a device shares that device's array: calls that overlap on
+ *   one device (other streams, other threads) race. */
+int fsx_cnn_conv_wgrad_noalloc(const void* x, int ci, const void* dy, int co_in, int B, int H,
+                               int W, float* dw, float* dbias, void* stream);
 int fsx_cnn_act(const float* y, long long npix, int cin, int cpad, int use_gelu, void* a,
                 void* stream);
 int fsx_cnn_act_bwd(const float* g, int gs, const float* y, long long npix, int c, void* d,
