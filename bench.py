@@ -784,15 +784,9 @@ def bench_closure(args, torch, _lib):
         return sim
 
     def run_until(sim, t):
-        ts = sim.time_stepping
-        if not getattr(ts, "_prepare_main_loop_called", False):
-            with contextlib.redirect_stdout(sys.stderr):
-                ts.prepare_main_loop()
-        n = 0
-        while ts.t < t - 1e-12:
-            ts.one_time_step()
-            n += 1
-        return n
+        # the solver's messages go to stderr: stdout is the JSON line
+        with contextlib.redirect_stdout(sys.stderr):
+            return CL.run_until(sim, t)
 
     Cls, pl = CL.ns2d_params(nl, nu, forcing=False)
     with contextlib.redirect_stdout(sys.stderr):
@@ -839,42 +833,13 @@ def bench_closure(args, torch, _lib):
             les.closure_net = net.folded(sw, s_s)
         runs[name] = les
     runs["dns"] = dns
-    # same forcing in the three runs: the coarse forcing box does not depend
-    # on the grid, so the LES makers take the DNS maker's random pair and
-    # renewal time, and each run draws its renewals from its own copy of
-    # the random state
-    src = dns.forcing.forcing_maker
-    for name in ("les_closure", "les_plain"):
-        mk = runs[name].forcing.forcing_maker
-        if mk.forcing0.shape != src.forcing0.shape:
-            raise ValueError("DNS and LES coarse forcing boxes differ")
-        mk.t_last_change = src.t_last_change
-        mk._seed0, mk._seed1 = src._seed0, src._seed1
-        mk.forcing0, mk.forcing1 = src.forcing0.copy(), src.forcing1.copy()
-        mk._upload_pair(mk.forcing0, mk.forcing1)
-    rng_state = {k: np.random.get_state() for k in runs}
-    spectra = {k: [] for k in runs}
-    wall = {k: 0.0 for k in runs}
-    steps = {k: 0 for k in runs}
-    t_start = dns.time_stepping.t
-    nsamp = int(round(args.closure_eval_t / args.closure_sample_dt))
-    for j in range(nsamp):
-        t_next = t_start + (j + 1) * args.closure_sample_dt
-        for k, sim in runs.items():
-            np.random.set_state(rng_state[k])
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            steps[k] += run_until(sim, t_next)
-            torch.cuda.synchronize()
-            wall[k] += time.perf_counter() - t0
-            rng_state[k] = np.random.get_state()
-            rot = sim.state.state_spect.data
-            if k == "dns":
-                rot = CL.cut_to_les(sim.oper, les_oper, rot[0])
-            spectra[k].append(CL.energy_spectrum(les_oper, rot))
-        for k, sim in runs.items():
-            if not np.isfinite(spectra[k][-1]).all():
-                raise ValueError(f"{k}: non-finite spectrum at t = {sim.time_stepping.t}")
+    # same forcing in the three runs; each draws its renewals from its own
+    # copy of the random state
+    CL.share_forcing(dns, [runs["les_closure"], runs["les_plain"]])
+    with contextlib.redirect_stdout(sys.stderr):
+        spectra, wall, steps = CL.compare_runs(runs, les_oper, args.closure_eval_t,
+                                               args.closure_sample_dt)
+    rng_state = {"les_closure": np.random.get_state()}
     # per-launch rate of the hidden-layer convolutions (HIP events on the
     # LES stream) over extra LES steps after the timed evaluation
     les = runs["les_closure"]

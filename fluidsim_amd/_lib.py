@@ -82,6 +82,8 @@ SIGNATURES = {
     "fsx_forcing_enable": [P, c_int],
     "fsx_forcing_step": [P, c_double, c_double, c_int],
     "fsx_forcing_normalize": [P, P, P],
+    "fsx_forcing_graph": [P, c_int],
+    "fsx_forcing_push": [P, P],
     "fsx_forcing_add": [P, P, c_int, c_double, P],
     "fsx_strain_rate": [P, P, P, P],
     "fsx_smagorinsky_phys": [P, P, c_double, P],

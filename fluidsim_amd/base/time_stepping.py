@@ -44,6 +44,9 @@ class TimeSteppingPseudoSpectral:
             "deltat_max": 0.2,
             "cfl_coef": None,
             "max_elapsed": None,
+            # added (no reference counterpart): replay the fused Runge-Kutta
+            # step from a HIP graph
+            "hip_graph": False,
         }
         params._set_child("time_stepping", attribs=attribs)
         params.time_stepping._set_child(
@@ -281,6 +284,8 @@ class TimeSteppingPseudoSpectral:
             self.compute_time_increment_CLF()
         if pending:
             self._push_dt()
+            if getattr(self, "_forcing_graph", False):
+                _lib.call("fsx_forcing_push", sim.oper.plan.handle, self._stream())
             _lib.call("fsx_forcing_normalize", sim.oper.plan.handle,
                       _lib.dptr(sim.state.state_spect.data), self._stream())
             maker._pending = False
@@ -344,10 +349,10 @@ class TimeSteppingPseudoSpectral:
         use_cfl = p.USE_CFL and not self._pre_done
         if fused and hasattr(sim, "_engine_rk_stage"):
             cfl = float(self.CFL) if use_cfl else 0.0
-            for stage in range(self._scheme):
-                sim._engine_rk_stage(self._scheme, stage, S, tmp, t1, self._visc,
-                                     cfl if stage == 0 else 0.0,
-                                     float(self.deltat_max))
+            if getattr(p, "hip_graph", False) and not self._pre_done:
+                self._graph_step(cfl, maker)
+            else:
+                self._issue_stages(cfl)
             if maker is not None:
                 maker._pending = False  # consumed by stage 0
             return
@@ -362,6 +367,8 @@ class TimeSteppingPseudoSpectral:
             _lib.call("fsx_cfl_update", plan, sim._cfl_ncomp, float(self.CFL),
                       float(self.deltat_max), s)
         if maker is not None and getattr(maker, "_pending", False):
+            if getattr(self, "_forcing_graph", False):
+                _lib.call("fsx_forcing_push", plan, s)
             _lib.call("fsx_forcing_normalize", plan, _lib.dptr(S), s)
             maker._pending = False
         for stage in range(self._scheme):
@@ -370,6 +377,72 @@ class TimeSteppingPseudoSpectral:
             _lib.call("fsx_rk_stage", plan, self._scheme, stage, nvar,
                       _lib.dptr(S), _lib.dptr(tmp), _lib.dptr(t1), _lib.dptr(N),
                       proj, novz, self._visc, s)
+
+    def _issue_stages(self, cfl):
+        sim = self.sim
+        S = sim.state.state_spect.data
+        tmp = self._state_spect_tmp.data
+        t1 = self._state_spect_tmp1.data
+        for stage in range(self._scheme):
+            sim._engine_rk_stage(self._scheme, stage, S, tmp, t1, self._visc,
+                                 cfl if stage == 0 else 0.0, float(self.deltat_max))
+
+    _gstream = None
+
+    def _graph_step(self, cfl, maker):
+        """The fused step replayed from a HIP graph (params.time_stepping
+        .hip_graph): the launches of the stages are captured once per
+        variant (CFL update in stage 0 or not, forcing, the solver's own key)
+        on a stream of the time stepping and replayed on it.  Everything a
+        step changes lives in device memory: dt (CFL rule), the state and
+        scratch arrays, the forcing box and, in graph mode, the forcing
+        normalisation's coefficients, pushed before each replay.  The first
+        step of a variant runs eagerly (kernel attributes, buffers the
+        stages allocate), the second is captured, then replayed."""
+        import torch
+
+        sim = self.sim
+        plan = sim.oper.plan.handle
+        if self._gstream is None:
+            self._gstream = torch.cuda.Stream()
+            self._graphs = {}
+        if maker is not None and not getattr(self, "_forcing_graph", False):
+            _lib.call("fsx_forcing_graph", plan, 1)
+            self._forcing_graph = True
+        key = (cfl > 0, maker is not None,
+               sim._graph_key() if hasattr(sim, "_graph_key") else None)
+        cur = torch.cuda.current_stream()
+        gs = self._gstream
+        gs.wait_stream(cur)
+        with torch.cuda.stream(gs):
+            sp = _lib.stream_ptr(gs)
+            if maker is not None:
+                _lib.call("fsx_forcing_push", plan, sp)
+            g = self._graphs.get(key)
+            if g is None:
+                self._issue_stages(cfl)
+                self._graphs[key] = "eager"
+            elif g == "eager":
+                h = ctypes.c_void_p()
+                _lib.call("fsx_graph_begin", sp)
+                try:
+                    self._issue_stages(cfl)
+                finally:
+                    rc = _lib.get_lib().fsx_graph_end(sp, ctypes.byref(h))
+                _lib.check(rc, "fsx_graph_end")
+                self._graphs[key] = h
+                _lib.call("fsx_graph_launch", h, sp)
+            else:
+                _lib.call("fsx_graph_launch", g, sp)
+        cur.wait_stream(gs)
+
+    def __del__(self):
+        for g in getattr(self, "_graphs", {}).values():
+            if isinstance(g, ctypes.c_void_p) and g.value:
+                try:
+                    _lib.call("fsx_graph_destroy", g)
+                except Exception:
+                    pass
 
     # ------------------------------------------------------------ generic
     def _combine(self, out, terms, shift=(0.0, 0.0, 0.0), check_nan=False):
@@ -408,6 +481,8 @@ class TimeSteppingPseudoSpectral:
         maker = (sim.forcing.forcing_maker
                  if getattr(sim, "is_forcing_enabled", False) else None)
         if maker is not None and getattr(maker, "_pending", False):
+            if getattr(self, "_forcing_graph", False):
+                _lib.call("fsx_forcing_push", plan, s)
             _lib.call("fsx_forcing_normalize", plan, _lib.dptr(sim.state.state_spect.data), s)
             maker._pending = False
         S = sim.state.state_spect

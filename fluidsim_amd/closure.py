@@ -289,6 +289,11 @@ def make_les_class():
     class SimulLES(SimulNS2D):
         closure_net = None  # CNNClosure with folded scales (raw w -> raw S)
 
+        def _graph_key(self):
+            """A captured step holds the network's weight and work buffers:
+            one graph per network object."""
+            return id(self.closure_net) if self.closure_net is not None else None
+
         def _closure_term(self, src):
             """Network term in spectral space from the vorticity src[0]: one
             engine call (c2r, network, r2c, dealiasing)."""
@@ -344,6 +349,78 @@ def ns2d_params(n, nu_2, precision="float32", forcing=True, nkmin=8, nkmax=10, r
         params.forcing.nkmax_forcing = nkmax
         params.forcing.forcing_rate = rate
     return cls, params
+
+
+def share_forcing(src, sims):
+    """Give the runs `sims` the forcing of `src`: the coarse forcing box does
+    not depend on the grid, so each maker takes src's random pair and renewal
+    time (fluidsim/base/forcing/specific.py:768-868); every run then draws
+    its renewals from its own copy of numpy's random state."""
+    mk0 = src.forcing.forcing_maker
+    for sim in sims:
+        mk = sim.forcing.forcing_maker
+        if mk.forcing0.shape != mk0.forcing0.shape:
+            raise ValueError("the runs' coarse forcing boxes differ")
+        mk.t_last_change = mk0.t_last_change
+        mk._seed0, mk._seed1 = mk0._seed0, mk0._seed1
+        mk.forcing0, mk.forcing1 = mk0.forcing0.copy(), mk0.forcing1.copy()
+        mk._upload_pair(mk.forcing0, mk.forcing1)
+
+
+def run_until(sim, t):
+    """Advance a Simul until its time reaches t; returns the steps taken."""
+    ts = sim.time_stepping
+    if not getattr(ts, "_prepare_main_loop_called", False):
+        ts.prepare_main_loop()
+    n = 0
+    while ts.t < t - 1e-12:
+        ts.one_time_step()
+        n += 1
+    return n
+
+
+def compare_runs(runs, les_oper, t_eval, sample_dt, dns_key="dns"):
+    """Advance the runs (name -> Simul, started at the same time) over t_eval
+    and sample their energy spectra on the LES grid every sample_dt (the
+    DNS cut to the LES grid); each run draws its forcing renewals from its
+    own copy of numpy's random state.  Returns the sampled spectra, the wall
+    time and the steps of each run."""
+    torch = _torch()
+    rng_state = {k: np.random.get_state() for k in runs}
+    spectra = {k: [] for k in runs}
+    wall = {k: 0.0 for k in runs}
+    steps = {k: 0 for k in runs}
+    import time as _time
+
+    t_start = next(iter(runs.values())).time_stepping.t
+    nsamp = int(round(t_eval / sample_dt))
+    for j in range(nsamp):
+        t_next = t_start + (j + 1) * sample_dt
+        for k, sim in runs.items():
+            np.random.set_state(rng_state[k])
+            torch.cuda.synchronize()
+            t0 = _time.perf_counter()
+            steps[k] += run_until(sim, t_next)
+            torch.cuda.synchronize()
+            wall[k] += _time.perf_counter() - t0
+            rng_state[k] = np.random.get_state()
+            rot = sim.state.state_spect.data
+            if k == dns_key:
+                rot = cut_to_les(sim.oper, les_oper, rot[0])
+            spectra[k].append(energy_spectrum(les_oper, rot))
+        for k, sim in runs.items():
+            if not np.isfinite(spectra[k][-1]).all():
+                raise ValueError(f"{k}: non-finite spectrum at t = {sim.time_stepping.t}")
+    return spectra, wall, steps
+
+
+def spectrum_errors(spectra, kmax, lo=0, dns_key="dns"):
+    """Relative L2 error of each run's time-averaged energy spectrum against
+    the DNS's over shells 1 + lo .. kmax."""
+    mean = {k: np.mean(v, axis=0)[1: kmax + 1] for k, v in spectra.items()}
+    ref = mean[dns_key][lo:]
+    return {k: float(np.linalg.norm(m[lo:] - ref) / np.linalg.norm(ref))
+            for k, m in mean.items() if k != dns_key}
 
 
 def run_steps(sim, nsteps):

@@ -416,6 +416,21 @@ int fsx_forcing_step(fsx_plan p, double coef, double forcing_rate, int which_roo
   return 0;
 }
 
+int fsx_forcing_graph(fsx_plan p, int on) {
+  if (!p) return fail(FSX_ERR_ARG, "bad argument");
+  p->fgraph = on != 0;
+  return 0;
+}
+
+int fsx_forcing_push(fsx_plan p, void* stream) {
+  if (!p) return fail(FSX_ERR_ARG, "bad argument");
+  if (!p->fgraph) return fail(FSX_ERR_ARG, "the plan's forcing is not in graph mode");
+  hipLaunchKernelGGL(k_forcing_par, dim3(1), dim3(64), 0, (hipStream_t)stream, p->d_scal + 16,
+                     p->fcoef, p->frate, (double)p->froot);
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
 int fsx_forcing_normalize(fsx_plan p, const void* S, void* stream) {
   PlanScope scope_(p);
   if (!p || !S) return fail(FSX_ERR_ARG, "bad argument");

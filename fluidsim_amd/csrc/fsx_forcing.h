@@ -44,7 +44,20 @@ struct ForcingArgs {
   double coef, rate;
   int which_root;        // 0 minabs, 1 first, 2 second, 3 positive
   const double* dt;
+  // when set: coef, rate and which_root of the step in device memory
+  // (par[0..2], written by k_forcing_par), so that a captured graph reads
+  // the values of each replay
+  const double* par;
 };
+
+// par[0..2] = coef, rate, which_root of the coming step
+static __global__ void k_forcing_par(double* par, double coef, double rate, double which_root) {
+  if (threadIdx.x == 0) {
+    par[0] = coef;
+    par[1] = rate;
+    par[2] = which_root;
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ void forced_mode(const ForcingArgs<T>& f, int m,
@@ -114,14 +127,15 @@ template <typename T>
 static __global__ __launch_bounds__(256) void k_forcing_norm(ForcingArgs<T> f) {
   __shared__ double red_a[256], red_b[256];
   __shared__ double s_R[2];
+  const double coef = f.par ? f.par[0] : f.coef;
   double pa = 0.0, pb = 0.0;
   for (int m = threadIdx.x; m < f.nmodes; m += blockDim.x) {
     int c, sy, cx;
     long long idx;
     forced_mode(f, m, c, idx, sy, cx);
     const cpx<double> f0 = f.f0[m], f1 = f.f1[m];
-    const cpx<double> fr = mk<double>(f1.x - f.coef * (f1.x - f0.x),
-                                      f1.y - f.coef * (f1.y - f0.y));
+    const cpx<double> fr = mk<double>(f1.x - coef * (f1.x - f0.x),
+                                      f1.y - coef * (f1.y - f0.y));
     cpx<double> v;
     if (f.key_kind == FKEY_VT) {
       const cpx<T> vx = f.S[idx], vy = f.S[f.ke + idx];
@@ -148,7 +162,9 @@ static __global__ __launch_bounds__(256) void k_forcing_norm(ForcingArgs<T> f) {
   }
   if (threadIdx.x == 0) {
     const double a = *f.dt / 2.0 * red_a[0];
-    const cpx<double> R = norm_root(a, red_b[0], -f.rate, f.which_root);
+    const double rate = f.par ? f.par[1] : f.rate;
+    const int which = f.par ? (int)f.par[2] : f.which_root;
+    const cpx<double> R = norm_root(a, red_b[0], -rate, which);
     s_R[0] = R.x;
     s_R[1] = R.y;
   }
@@ -159,8 +175,8 @@ static __global__ __launch_bounds__(256) void k_forcing_norm(ForcingArgs<T> f) {
     long long idx;
     forced_mode(f, m, c, idx, sy, cx);
     const cpx<double> f0 = f.f0[m], f1 = f.f1[m];
-    const cpx<double> fr = mk<double>(f1.x - f.coef * (f1.x - f0.x),
-                                      f1.y - f.coef * (f1.y - f0.y));
+    const cpx<double> fr = mk<double>(f1.x - coef * (f1.x - f0.x),
+                                      f1.y - coef * (f1.y - f0.y));
     const cpx<double> g = mulc(R, fr);
     if (f.key_kind == FKEY_VT) {
       double sp, cp;

@@ -92,7 +92,8 @@ struct fsx_plan_s {
   void* w2;
   size_t w1_elems, w2_elems;  // capacity in complex elements
   int nf_max;
-  double* d_scal;  // [0..3] maxima bits, [4] dt, [5] nan flag, [8..15] sums
+  double* d_scal;  // [0..3] maxima bits, [4] dt, [5] nan flag, [8..15] sums,
+                   // [16..18] forcing coef, rate, root (graph mode)
   // coarse forcing box (fsx_forcing_*)
   bool force_on, force_pending;
   void* d_fc;
@@ -106,6 +107,9 @@ struct fsx_plan_s {
   int fkey_kind, fkey_var;
   double fcoef, frate;
   int froot;
+  // the normalisation reads coef / rate / root from d_scal[16..18]
+  // (fsx_forcing_graph), pushed before each step by fsx_forcing_push
+  bool fgraph;
   // diagnostics: per-block records and the output (fsx_diag)
   double* d_dpart;
   size_t dpart_cap;
@@ -913,6 +917,7 @@ int forcing_normalize(fsx_plan_s* p, const void* S, hipStream_t s) {
   f.rate = p->frate;
   f.which_root = p->froot;
   f.dt = p->d_scal + 4;
+  f.par = p->fgraph ? p->d_scal + 16 : nullptr;
   hipLaunchKernelGGL(k_forcing_norm<T>, dim3(1), dim3(256), 0, s, f);
   FSX_HIP(hipGetLastError());
   return 0;

@@ -238,6 +238,15 @@ int fsx_forcing_enable(fsx_plan p, int on);
 int fsx_forcing_step(fsx_plan p, double coef, double forcing_rate, int which_root);
 int fsx_forcing_normalize(fsx_plan p, const void* S, void* stream);
 
+/* Graph mode of the normalisation (no reference counterpart: support for
+ * steps captured in a HIP graph, whose kernel arguments are fixed at
+ * capture).  fsx_forcing_graph(p, 1): the normalisation reads coef,
+ * forcing_rate and which_root from device memory; fsx_forcing_push writes
+ * the values of the last fsx_forcing_step there, stream-ordered (call it on
+ * the step's stream before each replay). */
+int fsx_forcing_graph(fsx_plan p, int on);
+int fsx_forcing_push(fsx_plan p, void* stream);
+
 /* out[v] += scale * Fc[v] for v < nvar, at the full-grid modes of the box
  * (put_coarse_array_in_array_fft, fluidsim/operators/operators2d.py
  * :525-600): the forcing added to tendencies_nonlin, or, on a zeroed array,
