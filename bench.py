@@ -696,7 +696,7 @@ def bench_ensemble(args, torch, _lib, copy_gbps):
     for g in ens.groups:
         _lib.call("fsx_timing_enable", g.handle, 1)
         _lib.call("fsx_timing_read", g.handle, m, b, c, 1)
-    ens.step(3, graph=False)
+    ens.step(3, graph=False, serial=True)
     ens.synchronize()
     for g in ens.groups:
         _lib.call("fsx_timing_read", g.handle, m, b, c, 1)
@@ -754,7 +754,8 @@ def bench_ensemble(args, torch, _lib, copy_gbps):
             "frac_of_copy": balg * steps_per_s / 1e9 / copy_gbps,
         },
         "kernels": kernels,
-        "tuning": {str(k): v["lines"] for k, v in ens.tuning.items()},
+        "tuning": {str(k): {"lines": v["lines"], "row_pass": v["row_pass"]}
+                   for k, v in ens.tuning.items()},
     }
     runs = ens.runs  # with each run's dt
     ens.close()

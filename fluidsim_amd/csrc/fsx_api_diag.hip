@@ -7,12 +7,10 @@ namespace {
 // LDS of one block record is kept within 64 KiB (two blocks per CU)
 constexpr int kDiagLdsDoubles = 8192;
 
-#ifndef FSX_DIAG_COPIES
-#define FSX_DIAG_COPIES 4
-#endif
-#ifndef FSX_DIAG_LDS
-#define FSX_DIAG_LDS (80 * 1024)
-#endif
+// histogram copies per block (lane parity spreads same-bin adds) and the
+// LDS budget of a block record with its copies (two blocks per CU)
+constexpr int kDiagCopies = 4;
+constexpr size_t kDiagLdsBytes = 80 * 1024;
 
 int diag_geom(const fsx_plan_s* p, int kind, int flags, DiagGeom& g, DiagOut& o,
               long long& total) {
@@ -87,12 +85,12 @@ int diag_record(DiagGeom& g, int q0, int nq) {
   g.off_sum = g.off_kh + ((g.flags & DIAG_KZKH) ? nq * g.nkh : 0);
   g.rec = g.off_sum + nq;
   // copies of the shell / kh histograms while the block's LDS stays within
-  // FSX_DIAG_LDS bytes (two blocks per CU at the default)
+  // kDiagLdsBytes (two blocks per CU)
   const int span = g.off_sum - g.off_k3;
   g.ncopy = 1;
   if (span > 0)
-    for (int c = FSX_DIAG_COPIES; c > 1; c >>= 1)
-      if ((size_t)(g.rec + (c - 1) * span) * sizeof(double) <= FSX_DIAG_LDS) {
+    for (int c = kDiagCopies; c > 1; c >>= 1)
+      if ((size_t)(g.rec + (c - 1) * span) * sizeof(double) <= kDiagLdsBytes) {
         g.ncopy = c;
         break;
       }

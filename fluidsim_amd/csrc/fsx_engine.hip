@@ -706,6 +706,15 @@ int fsx_batch_set_lines(fsx_plan p, int lines) {
   return 0;
 }
 
+
+int fsx_batch_set_row(fsx_plan p, int mode) {
+  if (!p) return fail(FSX_ERR_ARG, "bad argument");
+  if (p->nb < 1) return fail(FSX_ERR_ARG, "not a batch plan");
+  if (mode < 0 || mode > 2) return fail(FSX_ERR_ARG, "row mode must be 0, 1 or 2");
+  p->brow = mode;
+  return 0;
+}
+
 struct fsx_graph_s {
   hipGraphExec_t exec;
 };

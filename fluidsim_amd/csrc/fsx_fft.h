@@ -174,10 +174,7 @@ __host__ __device__ __forceinline__ int stage_log_radix(int logN, int logNs) {
 
 // LDS address with one pad slot per 32 elements: power-of-two strides of
 // the Stockham write patterns then fall on different banks.
-#ifndef FSX_LDS_PAD
-#define FSX_LDS_PAD 1
-#endif
-__device__ __forceinline__ int lpad(int a) { return FSX_LDS_PAD ? a + (a >> 5) : a; }
+__device__ __forceinline__ int lpad(int a) { return a + (a >> 5); }
 
 // Apply the Stockham twiddles W_{Ns R}^{r k} to v[1..R-1].
 // tw is a table exp(-2 pi i m / Ntab) with Ntab = N << twshift.

@@ -121,6 +121,7 @@ struct fsx_plan_s {
   int nb;
   double* d_bpar;
   int blines;
+  int brow;  // row pass of the batch (fsx_batch_set_row)
 };
 
 namespace {
@@ -181,19 +182,11 @@ int tm_flush(fsx_plan_s* p) {
 }
 
 constexpr int kThr = 512;   // threads per workgroup of the strided passes
-#ifndef FSX_XPASS_THREADS
-#define FSX_XPASS_THREADS 512
-#endif
-#ifndef FSX_XPASS_EPT_DIV
-#define FSX_XPASS_EPT_DIV 2
-#endif
-constexpr int kXThr = FSX_XPASS_THREADS;  // threads per workgroup of the row pass
+constexpr int kXThr = 512;       // threads per workgroup of the LDS row pass
+constexpr int kXPassEptDiv = 2;  // its tile: half the strided passes' elements per thread
 constexpr int kEPT32 = 16;  // 8192 complex64 per tile  = 64 KiB of LDS
 constexpr int kEPT64 = 8;   // 4096 complex128 per tile = 64 KiB of LDS
-#ifndef FSX_MIN_BLOCKS
-#define FSX_MIN_BLOCKS 1024
-#endif
-constexpr int kMinBlocks = FSX_MIN_BLOCKS;  // shrink tiles of small problems below this
+constexpr int kMinBlocks = 1024;  // shrink tiles of small problems below this
 
 template <typename T>
 constexpr int ept() {
@@ -331,7 +324,9 @@ int launch_strided(Op& op, int N, int nouter, int nlines, int nf, const void* tw
     return launch_strided_tile<T, DIR, E, kThr, Op, true>(op, N, need, blocks, nfb, s);
   }
   if (!wide && need * line_mul<Op>::value <= (size_t)E * kThr) need *= line_mul<Op>::value;
-  if (wide) return launch_strided_tile<T, DIR, E, KW>(op, N, need, blocks, nfb, s);
+  // 1024-thread tiles are FP32 only (wide_ok): no FP64 instantiation
+  if constexpr (sizeof(T) == 4)
+    if (wide) return launch_strided_tile<T, DIR, E, KW>(op, N, need, blocks, nfb, s);
   // small problems (few lines per tile): as many threads as the tile needs
   if ((size_t)E * kThr / 4 >= need)
     return launch_strided_tile<T, DIR, E, kThr / 4>(op, N, need, blocks, nfb, s);
@@ -345,7 +340,7 @@ template <typename T, class Op>
 int launch_xpass_gen(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
                      int ld_in, int mx_out, int ld_out, long long ifs, long long ofs,
                      T scl, const void* tw, hipStream_t s) {
-  constexpr int E = ept<T>() / FSX_XPASS_EPT_DIV >= 8 ? ept<T>() / FSX_XPASS_EPT_DIV : 8;
+  constexpr int E = ept<T>() / kXPassEptDiv >= 8 ? ept<T>() / kXPassEptDiv : 8;
   (void)E;
   // two buffers of complex rows in the workgroup's LDS, after the twiddles
   const int tile = (int)((kLdsMax / sizeof(cpx<T>) - (size_t)nx) / 2);
@@ -399,7 +394,7 @@ int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
   if (!is_pow2(nx) || nx < 4)
     return launch_xpass_gen<T>(op, nx, nrows, nf_in, nf_out, mx_in, ld_in, mx_out, ld_out,
                                ifs, ofs, scl, tw, s);
-  constexpr int E = ept<T>() / FSX_XPASS_EPT_DIV >= 8 ? ept<T>() / FSX_XPASS_EPT_DIV : 8;
+  constexpr int E = ept<T>() / kXPassEptDiv >= 8 ? ept<T>() / kXPassEptDiv : 8;
   const int tile = E * kXThr;
   const int M = nx / 2;
   const int nfm = nf_in > nf_out ? nf_in : nf_out;
@@ -445,34 +440,19 @@ int launch_xpass(Op& op, int nx, int nrows, int nf_in, int nf_out, int mx_in,
 // Register-resident fused row pass (k_xrow): 2^LOGP points per lane
 // (8 in FP32, 4 in FP64), a row within one wave.  Returns 1 when the row
 // length is outside what it covers (the caller then uses k_xpass).
-#ifndef FSX_XROW
-#define FSX_XROW 1
-#endif
-#ifndef FSX_XROW_THREADS
-#define FSX_XROW_THREADS 256
-#endif
-#ifndef FSX_XROW_BLOCKS_PER_CU
-#define FSX_XROW_BLOCKS_PER_CU 4
-#endif
-constexpr int kXRThr = FSX_XROW_THREADS;
+constexpr int kXRThr = 256;          // threads per workgroup
+constexpr int kXRBlocksPerCU = 4;    // grid cap: workgroups per CU
 
-#ifndef FSX_XR_LOGP32
-#define FSX_XR_LOGP32 3
-#endif
-#ifndef FSX_XR_LOGP64
-#define FSX_XR_LOGP64 2
-#endif
 template <typename T>
 constexpr int xrow_logp() {
-  return sizeof(T) == 4 ? FSX_XR_LOGP32 : FSX_XR_LOGP64;
+  return sizeof(T) == 4 ? 3 : 2;
 }
 
-template <typename T, int LOGM, class Op>
+template <typename T, int LOGM, class Op, int LOGP = xrow_logp<T>()>
 int launch_xrow_m(const XRDesc<T>& d, const Op& op, hipStream_t s) {
-  constexpr int LOGP = xrow_logp<T>();
   using PL = RowPlan<LOGM, LOGP>;
   constexpr int RPB = (kXRThr / 64) * (64 / PL::G);
-  const size_t lds = ((size_t)RPB * xrow_ld<T>(PL::M) + (FSX_XR_TWLDS ? 2 * PL::M : 0)) *
+  const size_t lds = ((size_t)RPB * xrow_ld<T>(PL::M) + 0) *
                      sizeof(cpx<T>);
   auto kern = k_xrow<T, LOGM, LOGP, kXRThr, Op>;
   static size_t lds_set = 0;
@@ -490,7 +470,7 @@ int launch_xrow_m(const XRDesc<T>& d, const Op& op, hipStream_t s) {
     if (ncu <= 0) ncu = 256;
   }
   const long long need = (d.nrows + RPB - 1) / RPB;
-  const long long cap = (long long)ncu * FSX_XROW_BLOCKS_PER_CU;
+  const long long cap = (long long)ncu * kXRBlocksPerCU;
   const unsigned blocks = (unsigned)(need < cap ? need : cap);
   if (d.nrows <= 0) return 0;
   int rc = tm_pre(s);
@@ -504,8 +484,8 @@ template <typename T, class Op>
 int launch_xrow(int nx, int nrows, int mx_in, int ld_in, int mx_out, int ld_out,
                 long long ifs, long long ofs, T scl, const void* in, void* out,
                 const void* tw, unsigned long long* maxima, const Op& op,
-                hipStream_t s) {
-  if (!FSX_XROW || !is_pow2(nx)) return 1;
+                hipStream_t s, bool radix4 = false) {
+  if (!is_pow2(nx)) return 1;
   XRDesc<T> d;
   d.nrows = nrows;
   d.mx_in = mx_in;
@@ -519,6 +499,22 @@ int launch_xrow(int nx, int nrows, int mx_in, int ld_in, int mx_out, int ld_out,
   d.out = reinterpret_cast<cpx<T>*>(out);
   d.tw = reinterpret_cast<const cpx<T>*>(tw);
   d.maxima = maxima;
+  if constexpr (sizeof(T) == 4 && std::is_same<Op, XRNS2D<T>>::value) {
+    // ns2d rows with 4 points per lane (radix-4 stages), a choice of the
+    // ensemble autotuner
+    if (radix4) {
+      switch (ilog2(nx / 2) - 2) {
+        case 0: return launch_xrow_m<T, 2, Op, 2>(d, op, s);
+        case 1: return launch_xrow_m<T, 3, Op, 2>(d, op, s);
+        case 2: return launch_xrow_m<T, 4, Op, 2>(d, op, s);
+        case 3: return launch_xrow_m<T, 5, Op, 2>(d, op, s);
+        case 4: return launch_xrow_m<T, 6, Op, 2>(d, op, s);
+        case 5: return launch_xrow_m<T, 7, Op, 2>(d, op, s);
+        case 6: return launch_xrow_m<T, 8, Op, 2>(d, op, s);
+        default: return 1;
+      }
+    }
+  }
   constexpr int LOGP = xrow_logp<T>();
   switch (ilog2(nx / 2) - LOGP) {
     case 0: return launch_xrow_m<T, LOGP + 0>(d, op, s);
@@ -1042,8 +1038,13 @@ int do_ns2d_batch_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp, v
   note(K_XPASS, 5.0 * rows * g.mx * sizeof(cpx<T>));
   XRNS2D<T> xr;
   xr.beta = (T)beta;
-  rc = launch_xrow<T>(p->nx, rows, g.mx, g.ld2, g.mx, g.ld2, fs2, fs2, T(1.0 / npts),
-                      W2<T>(p), W2<T>(p), p->tw[0], nullptr, xr, s);
+  // row pass of the batch (fsx_batch_set_row): 0 register rows (8 points
+  // per lane in FP32), 1 register rows with 4 points per lane, 2 the LDS
+  // row pass
+  rc = p->brow == 2 ? 1
+                    : launch_xrow<T>(p->nx, rows, g.mx, g.ld2, g.mx, g.ld2, fs2, fs2,
+                                     T(1.0 / npts), W2<T>(p), W2<T>(p), p->tw[0], nullptr, xr,
+                                     s, p->brow == 1);
   if (rc == 1) {
     OpXNS2D<T, kXThr> ox;
     ox.in = W2<T>(p);

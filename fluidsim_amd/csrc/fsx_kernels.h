@@ -200,45 +200,27 @@ __device__ __forceinline__ bool last_slot(int e, const StridedDesc<T>& d, int lo
 // Elements per thread of the multi-transform z-passes (tile = kThr x EPT /
 // div): the curl pass stashes 3 and the Runge-Kutta passes 3-4 transforms
 // in registers, so their tiles are halved by default.
-#ifndef FSX_ZCURL_EPTDIV
-#define FSX_ZCURL_EPTDIV 2
-#endif
-#ifndef FSX_ZRK_EPTDIV
-#define FSX_ZRK_EPTDIV 2
-#endif
+constexpr int kZCurlEptDiv = 2;
+constexpr int kZRKEptDiv = 2;
 
 // ---------------------------------------------------------------- strided
 // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on
-// XCD b mod FSX_NXCD), each XCD with its own L2.  In passes over K rows
+// XCD b mod kNXCD), each XCD with its own L2.  In passes over K rows
 // (nkx = nx/2 + 1 complex values, not line-aligned), and in passes whose
 // tiles hold runs shorter than a 128-byte line (long transforms leave few
 // lines per tile), neighbouring line chunks share cache lines, so there
 // consecutive tiles are given to the same XCD: XCD x takes the x-th
 // contiguous range.
-#ifndef FSX_NXCD
-#define FSX_NXCD 8
-#endif
+constexpr int kNXCD = 8;
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
-  if (FSX_NXCD <= 1 || nb < FSX_NXCD) return b;
-  const int x = b % FSX_NXCD, local = b / FSX_NXCD;
-  const int q = nb / FSX_NXCD, rem = nb % FSX_NXCD;
+  if (nb < kNXCD) return b;
+  const int x = b % kNXCD, local = b / kNXCD;
+  const int q = nb / kNXCD, rem = nb % kNXCD;
   return x < rem ? x * (q + 1) + local : rem * (q + 1) + (x - rem) * q + local;
 }
 
-// Minimum waves per SIMD asked of the strided passes (0: compiler's
-// choice).  Only the multi-transform z-passes use more registers than
-// this allows by default.
-#ifndef FSX_STRIDED_WAVES
-#define FSX_STRIDED_WAVES 0
-#endif
-#if FSX_STRIDED_WAVES > 0
-#define FSX_STRIDED_ATTR __attribute__((amdgpu_waves_per_eu(FSX_STRIDED_WAVES)))
-#else
-#define FSX_STRIDED_ATTR
-#endif
-
 template <typename T, int DIR, int EPT, int kT, class Op, bool GEN = false>
-static __global__ __launch_bounds__(kT) FSX_STRIDED_ATTR void k_strided(Op op) {
+static __global__ __launch_bounds__(kT) void k_strided(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const StridedDesc<T>& d = op.d;
   cpx<T>* s_tw = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -401,7 +383,7 @@ struct OpYFwd2 : PlainRun<T, OpYFwd2<T>> {
 // each input read from HBM once; outputs 0-2 are v, 3-5 omega.
 template <typename T>
 struct OpZInvCurl {
-  static constexpr int kEptDiv = FSX_ZCURL_EPTDIV;  // register stashes: half-size tiles
+  static constexpr int kEptDiv = kZCurlEptDiv;  // register stashes: half-size tiles
   static constexpr bool kFieldPerBlock = false;
   StridedDesc<T> d;
   const cpx<T>* in;  // 3 fields
@@ -580,7 +562,7 @@ struct OpYInv2NS {
 // (rk_point); the tendency never goes to HBM.
 template <typename T>
 struct OpZFwdRK3 {
-  static constexpr int kEptDiv = FSX_ZRK_EPTDIV;
+  static constexpr int kEptDiv = kZRKEptDiv;
   static constexpr bool kFieldPerBlock = false;
   StridedDesc<T> d;
   const cpx<T>* in;  // W1, 3 fields
@@ -634,7 +616,7 @@ struct OpZFwdRK3 {
 // projection acts on N_v only (rk_point).
 template <typename T>
 struct OpZFwdRKStrat {
-  static constexpr int kEptDiv = FSX_ZRK_EPTDIV;
+  static constexpr int kEptDiv = kZRKEptDiv;
   static constexpr bool kFieldPerBlock = false;
   StridedDesc<T> d;
   const cpx<T>* in;  // W1, 6 fields
@@ -800,12 +782,9 @@ __device__ __forceinline__ int rpad(int lr, int x, int ldr) {
   return 2 * (lr * ldr + lpad(x >> 1)) + (x & 1);
 }
 
-#ifndef FSX_XPASS_WAVES
-#define FSX_XPASS_WAVES 4
-#endif
-// At least FSX_XPASS_WAVES waves per SIMD (caps the VGPR count).
+// At least 4 waves per SIMD (caps the VGPR count).
 template <typename T, int EPT, int kT, class Op>
-static __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XPASS_WAVES)))
+static __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_xpass(Op op) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const XDesc<T>& d = op.d;

@@ -74,21 +74,17 @@ struct RowPlan {
   }
 };
 
-// Padded LDS position in a row buffer: one slot per 2^FSX_XR_PAD elements.
-#ifndef FSX_XR_PAD32
-#define FSX_XR_PAD32 3
-#endif
-#ifndef FSX_XR_PAD64
-#define FSX_XR_PAD64 2
-#endif
+// Padded LDS position in a row buffer: one slot per 8 (FP32) or 4 (FP64)
+// elements, so the Stockham writes of stride 8 spread over the banks.
+template <typename T>
+constexpr int xr_pad_shift() { return sizeof(T) == 4 ? 3 : 2; }
 template <typename T>
 __device__ __forceinline__ int xpad(int a) {
-  constexpr int sh = sizeof(T) == 4 ? FSX_XR_PAD32 : FSX_XR_PAD64;
-  return a + (a >> sh);
+  return a + (a >> xr_pad_shift<T>());
 }
 template <typename T>
 __host__ __device__ constexpr int xrow_ld(int M) {
-  return M + (M >> (sizeof(T) == 4 ? FSX_XR_PAD32 : FSX_XR_PAD64));
+  return M + (M >> xr_pad_shift<T>());
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -97,40 +93,26 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Stage twiddles from registers (0) or from an LDS copy of the table (1).
-#ifndef FSX_XR_TWLDS
-#define FSX_XR_TWLDS 0
-#endif
-
 // Lane twiddles: W^k for the c2r/r2c pairing (k = l + G q) and the Stockham
 // twiddles W_{Ns R}^{r k} of stages 1.. (forward sign; conjugated for the
-// inverse).  tw = exp(-2 pi i m / (2M)), m < 2M.  With FSX_XR_TWLDS the
-// stage twiddles are read from the LDS table tl instead.
+// inverse).  tw = exp(-2 pi i m / (2M)), m < 2M.
 template <typename T, int LOGM, int LOGP>
 struct LaneTw {
   using PL = RowPlan<LOGM, LOGP>;
-  static constexpr int NST = FSX_XR_TWLDS ? 1 : (PL::NTW > 0 ? PL::NTW : 1);
+  static constexpr int NST = PL::NTW > 0 ? PL::NTW : 1;
   cpx<T> pair[PL::P];
   cpx<T> st[NST];
-  const cpx<T>* tl;
 
   __device__ __forceinline__ void load(const cpx<T>* tw, int l) {
 #pragma unroll
     for (int q = 0; q < PL::P; ++q) pair[q] = tw[l + PL::G * q];
-    if constexpr (PL::nstages() > 1 && !FSX_XR_TWLDS) load_stage<1>(tw, l);
+    if constexpr (PL::nstages() > 1) load_stage<1>(tw, l);
   }
   // stage-S twiddle r of butterfly t of lane l
   template <int S>
   __device__ __forceinline__ cpx<T> get(int l, int t, int r) const {
-    if constexpr (FSX_XR_TWLDS) {
-      constexpr int lr = PL::logr(S), lns = PL::lognsof(S);
-      constexpr int sh = (LOGM + 1) - (lns + lr);
-      const int k = (l + PL::G * t) & ((1 << lns) - 1);
-      return tl[(r * k) << sh];
-    } else {
-      constexpr int R = 1 << PL::logr(S);
-      return st[PL::ntw(S) + t * (R - 1) + (r - 1)];
-    }
+    constexpr int R = 1 << PL::logr(S);
+    return st[PL::ntw(S) + t * (R - 1) + (r - 1)];
   }
   // W_{Ns R}^{r k} = tw[r k 2M / (Ns R)] for the lane's butterflies of
   // stage S, stored from offset ntw(S)
@@ -324,16 +306,13 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, off, 0, 0);
 }
 
-#ifndef FSX_XR_WAVES
-#define FSX_XR_WAVES 2
-#endif
 
 // Fused row pass, kT threads per workgroup (kT/64 waves, 64/G rows per
 // wave), grid-stride over rows.  LDS: one row buffer per row of a wave.
 // Global accesses go through one buffer resource per (wave, field) that
 // spans the wave's valid rows.
 template <typename T, int LOGM, int LOGP, int kT, class Op>
-static __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(FSX_XR_WAVES)))
+static __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(2)))
 void k_xrow(XRDesc<T> d, Op op) {
   using PL = RowPlan<LOGM, LOGP>;
   constexpr int P = PL::P, G = PL::G, M = PL::M;
@@ -349,13 +328,6 @@ void k_xrow(XRDesc<T> d, Op op) {
 
   LaneTw<T, LOGM, LOGP> tw;
   tw.load(d.tw, l);
-  if constexpr (FSX_XR_TWLDS) {
-    // table after the row buffers
-    cpx<T>* tl = reinterpret_cast<cpx<T>*>(smem_raw) + RPB * LD;
-    for (int i = threadIdx.x; i < 2 * M; i += kT) tl[i] = d.tw[i];
-    __syncthreads();
-    tw.tl = tl;
-  }
 
   // per-lane byte offsets within the wave's rows (kOOB where masked)
   const unsigned rin = (unsigned)rw * d.ld_in * cs, rout = (unsigned)rw * d.ld_out * cs;

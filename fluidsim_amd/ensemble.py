@@ -10,11 +10,15 @@ Runs of one grid size form a group: one batch plan
 (``fsx_plan_create_batch``) whose state arrays hold the group's runs one
 after the other, (nb, n, n/2 + 1), and whose passes take the run as their
 outer index, so a Runge-Kutta stage of every run of the group is three
-launches.  A step of the whole ensemble (all groups, all stages) is
-captured once into a HIP graph and replayed (``fsx_graph_*``).  The lines
-per tile of the batched y-passes are picked by a small autotuner that times
-the candidates on the group itself and caches the choice in a JSON file
-keyed by (size, batch, precision, device).
+launches.  A step of the whole ensemble is captured once into a HIP graph
+and replayed (``fsx_graph_*``): the groups are parallel branches of the
+graph (each group's stages on a stream of its own, forked from and joined
+back to the capturing stream by events), so the launches of different
+sizes overlap on the GPU.  A small autotuner picks, per group, the lines
+per tile of the batched y-passes and the row pass (register rows with
+radix-8 or radix-4 stages, or the LDS Stockham row pass) by timing every
+pair on the group itself, and caches the choice in a JSON file keyed by
+(size, batch, precision, scheme, device).
 
 Each run's fixed dt is the initial CFL time step of fluidsim
 (fluidsim/base/time_stepping/base.py:320-366: CFL / (max|u|/dx + max|v|/dy),
@@ -71,6 +75,8 @@ class _Group:
         self.tmp = torch.zeros_like(self.S)
         self.t1 = torch.zeros_like(self.S)
         self.lines = 0
+        self.row = 0
+        self.stream = None  # the branch stream of the group (set by the ensemble)
 
     def close(self):
         if self.handle:
@@ -115,6 +121,10 @@ class EnsembleNS2D:
             for pos, i in enumerate(idx):
                 self.where[i] = (g, pos)
             self.groups.append(g)
+        # one branch stream per group beyond the first (default priority)
+        self.groups[0].stream = self.stream
+        for g in self.groups[1:]:
+            g.stream = torch.cuda.Stream()
         # the groups' buffers were zero-filled on the current stream
         self.stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(self.stream):
@@ -161,12 +171,33 @@ class EnsembleNS2D:
         return out[4]
 
     # ------------------------------------------------------------ steps
-    def _issue_step(self):
-        scheme = self.nstages
-        for g in self.groups:
-            for stage in range(scheme):
-                _lib.call("fsx_ns2d_batch_rk_stage", g.handle, scheme, stage, _lib.dptr(g.S),
-                          _lib.dptr(g.tmp), _lib.dptr(g.t1), self.beta, self._sp)
+    def _issue_group(self, g, sp):
+        for stage in range(self.nstages):
+            _lib.call("fsx_ns2d_batch_rk_stage", g.handle, self.nstages, stage, _lib.dptr(g.S),
+                      _lib.dptr(g.tmp), _lib.dptr(g.t1), self.beta, sp)
+
+    def _issue_step(self, serial=False):
+        """One step of every group: the first group on the ensemble's
+        stream, the others on their branch streams, forked from it and
+        joined back to it by events (under capture: parallel graph
+        branches); serial: every group on the ensemble's stream."""
+        import torch
+
+        if serial:
+            for g in self.groups:
+                self._issue_group(g, self._sp)
+            return
+        main = self.stream
+        fork = torch.cuda.Event()
+        fork.record(main)
+        for g in self.groups[1:]:
+            g.stream.wait_event(fork)
+            self._issue_group(g, _lib.stream_ptr(g.stream))
+        self._issue_group(self.groups[0], self._sp)
+        for g in self.groups[1:]:
+            join = torch.cuda.Event()
+            join.record(g.stream)
+            main.wait_event(join)
 
     def _capture(self):
         import torch
@@ -182,8 +213,10 @@ class EnsembleNS2D:
         self._graph = h
         torch.cuda.synchronize()
 
-    def step(self, nsteps=1, graph=True):
-        """Advance every run by nsteps steps (one graph launch per step)."""
+    def step(self, nsteps=1, graph=True, serial=False):
+        """Advance every run by nsteps steps (one graph launch per step);
+        without the graph, serial issues the groups one after the other on
+        one stream (per-launch timing)."""
         if nsteps <= 0:
             return
         if graph and self._graph is None:
@@ -196,7 +229,7 @@ class EnsembleNS2D:
             if graph:
                 _lib.call("fsx_graph_launch", self._graph, self._sp)
             else:
-                self._issue_step()
+                self._issue_step(serial=serial)
             self.it += 1
 
     def synchronize(self):
@@ -229,26 +262,24 @@ class EnsembleNS2D:
         e1 = torch.cuda.Event(enable_timing=True)
         with torch.cuda.stream(self.stream):
             S0 = g.S.clone()
-            for stage in range(self.nstages):  # warm
-                _lib.call("fsx_ns2d_batch_rk_stage", g.handle, self.nstages, stage,
-                          _lib.dptr(g.S), _lib.dptr(g.tmp), _lib.dptr(g.t1), self.beta,
-                          self._sp)
+            self._issue_group(g, self._sp)  # warm
             e0.record(self.stream)
             for _ in range(reps):
-                for stage in range(self.nstages):
-                    _lib.call("fsx_ns2d_batch_rk_stage", g.handle, self.nstages, stage,
-                              _lib.dptr(g.S), _lib.dptr(g.tmp), _lib.dptr(g.t1),
-                              self.beta, self._sp)
+                self._issue_group(g, self._sp)
             e1.record(self.stream)
             g.S.copy_(S0)
         self.stream.synchronize()
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    # row passes the autotuner tries (fsx_batch_set_row): register rows with
+    # radix-8 stages, with radix-4 stages (FP32 only), the LDS row pass
+    ROW_MODES = {0: "register radix-8", 1: "register radix-4", 2: "LDS radix 8/4/2"}
+
     def autotune(self, cache=None):
-        """Pick the lines per tile of each group's y-passes by timing its
-        RK4 step; choices cached in JSON keyed by size, batch, precision
-        and device name."""
+        """Pick, per group, the lines per tile of the y-passes and the row
+        pass by timing the group's RK step for every pair; choices cached in
+        JSON keyed by size, batch, precision, scheme and device name."""
         import torch
 
         path = cache or os.environ.get("FLUIDSIM_AMD_TUNE_CACHE", _DEFAULT_CACHE)
@@ -260,24 +291,28 @@ class EnsembleNS2D:
         dev = torch.cuda.get_device_name(0)
         changed = False
         self.tuning = {}
+        rows = [0, 1, 2] if self.precision == "float32" else [0, 2]
         for g in self.groups:
-            key = f"{dev}|ns2d|{g.n}|{g.nb}|{self.precision}|{self.nstages}"
+            key = f"{dev}|ns2d-v2|{g.n}|{g.nb}|{self.precision}|{self.nstages}"
             if key in table:
-                lines = int(table[key]["lines"])
+                lines, row = int(table[key]["lines"]), int(table[key]["row"])
             else:
-                best, lines, times = None, 0, {}
+                best, lines, row, times = None, 0, 0, {}
                 cands = [0] + [1 << k for k in range(0, 8) if (1 << k) <= g.n]
-                for cand in cands:
-                    _lib.call("fsx_batch_set_lines", g.handle, cand)
-                    ms = self._time_group(g)
-                    times[cand] = ms
-                    if best is None or ms < best:
-                        best, lines = ms, cand
-                table[key] = {"lines": lines, "ms_per_step": best,
-                              "candidates": {str(k): v for k, v in times.items()}}
+                for r in rows:
+                    _lib.call("fsx_batch_set_row", g.handle, r)
+                    for cand in cands:
+                        _lib.call("fsx_batch_set_lines", g.handle, cand)
+                        ms = self._time_group(g)
+                        times[f"{cand}/{r}"] = ms
+                        if best is None or ms < best:
+                            best, lines, row = ms, cand, r
+                table[key] = {"lines": lines, "row": row, "row_pass": self.ROW_MODES[row],
+                              "ms_per_step": best, "candidates (lines/row)": times}
                 changed = True
             _lib.call("fsx_batch_set_lines", g.handle, lines)
-            g.lines = lines
+            _lib.call("fsx_batch_set_row", g.handle, row)
+            g.lines, g.row = lines, row
             self.tuning[g.n] = table[key]
         if changed:
             try:

@@ -366,6 +366,11 @@ int fsx_plan_create_batch(fsx_plan* out, int nx, int ny, double Lx, double Ly,
                           int nbatch);
 int fsx_batch_set_params(fsx_plan p, const double* dt, const double* visc4);
 int fsx_batch_set_lines(fsx_plan p, int lines);
+/* fsx_batch_set_row: row pass of the batched stages, a choice of the
+ * ensemble autotuner: 0 register rows with radix-8 stages (8 points per
+ * lane in FP32; default), 1 register rows with radix-4 stages (FP32), 2 the
+ * LDS row pass (Stockham radix 8/4/2 in LDS). */
+int fsx_batch_set_row(fsx_plan p, int mode);
 int fsx_ns2d_batch_rk_stage(fsx_plan p, int scheme, int stage, void* S, void* tmp, void* t1,
                             double beta, void* stream);
 int fsx_graph_begin(void* stream);

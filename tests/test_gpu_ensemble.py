@@ -14,6 +14,7 @@ import json
 import numpy as np
 import pytest
 
+from fluidsim_amd import _lib
 from oracle.operators import Oper2D
 from oracle.solvers import NS2D
 from fluidsim_amd.synthetic import ns2d_noise_rot_fft
@@ -89,18 +90,52 @@ def test_autotune_cache_and_tile_independence(tmp_path):
     a = EnsembleNS2D(runs, precision="float32", tune_cache=str(cache))
     table = json.loads(cache.read_text())
     (key, entry), = table.items()
-    assert "|ns2d|64|4|float32|4" in key and "candidates" in entry
-    assert a.tuning[64]["lines"] == entry["lines"]
-    # the cached choice is reused, and the tile width does not change results
+    assert "|ns2d-v2|64|4|float32|4" in key and "candidates (lines/row)" in entry
+    assert len(entry["candidates (lines/row)"]) == 3 * 8  # lines 0, 1..64 x 3 row passes
+    assert a.tuning[64]["lines"] == entry["lines"] and a.tuning[64]["row"] == entry["row"]
+    # the cached choice is reused
     b = EnsembleNS2D(runs, precision="float32", tune_cache=str(cache))
-    assert b.groups[0].lines == a.groups[0].lines
+    assert (b.groups[0].lines, b.groups[0].row) == (a.groups[0].lines, a.groups[0].row)
+    # the tile width does not change results; the row pass changes FP32
+    # rounding only
     c = EnsembleNS2D(runs, precision="float32", autotune=False)
+    _lib.call("fsx_batch_set_row", c.groups[0].handle, a.groups[0].row)
     for e in (a, c):
         e.step(4, graph=False)
     for i in range(len(runs)):
         assert np.array_equal(a.get_state(i), c.get_state(i)), i
     for e in (a, b, c):
         e.close()
+
+
+@pytest.mark.parametrize("row", [0, 1, 2])
+def test_row_passes_vs_oracle(row, tmp_path):
+    """Each row pass the autotuner may pick (register rows with radix-8 or
+    radix-4 stages, the LDS row pass): FP32 runs of two sizes against the
+    FP64 oracle after 20 RK4 steps (1e-4, as the cfg-3 subset), and FP64
+    runs (row passes 0 and 2; 1 is FP32-only) within 1e-12."""
+    from fluidsim_amd.ensemble import EnsembleNS2D
+
+    print(f"case ensemble row pass {row}", flush=True)
+    for prec, tol in (("float32", 1e-4), ("float64", 1e-12)):
+        if prec == "float64" and row == 1:
+            continue
+        runs = make_runs([(64, 2), (128, 1)], seed0=50)
+        ens = EnsembleNS2D(runs, precision=prec, autotune=False)
+        for g in ens.groups:
+            _lib.call("fsx_batch_set_row", g.handle, row)
+        S0 = [ens.get_state(i).astype(np.complex128) for i in range(len(runs))]
+        ens.step(20)
+        ens.check_nan()
+        for i, r in enumerate(runs):
+            n = r["n"]
+            orc = NS2D(Oper2D(n, n), nu_2=r["nu_2"], deltat0=ens.runs[i]["deltat"])
+            orc.state_spect[0] = S0[i]
+            orc.run(20)
+            err = rel_l2(ens.get_state(i), orc.state_spect[0])
+            print(f"row pass {row} {prec} run {i} n={n}: rel L2 = {err:.3e}")
+            assert err <= tol
+        ens.close()
 
 
 def test_cfg3_subset_fp32_error(tmp_path):
