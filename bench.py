@@ -351,6 +351,25 @@ def bench_ns2d(args, torch, _lib, copy_gbps):
     _lib.call("fsx_timing_read", plan, ms8, by8, cn8, 1)
     _lib.call("fsx_timing_enable", plan, 0)
     steps_per_s = args.steps / elapsed
+    # the same run with its steps replayed from a HIP graph
+    del sim, ts
+    gc.collect()
+    Cls, params = CL.ns2d_params(n, 1e-5, precision=args.precision, graph=True)
+    np.random.seed(0)
+    with contextlib.redirect_stdout(sys.stderr):
+        sim = Cls(params)
+        sim.state.init_statespect_from(
+            rot_fft=ns2d_noise_rot_fft(n, n, seed=0, kmax=12, velo_max=0.5))
+        ts = sim.time_stepping
+        ts.prepare_main_loop()
+    for _ in range(max(args.warmup, 3)):
+        ts.one_time_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts.one_time_step()
+    torch.cuda.synchronize()
+    graph_steps_per_s = args.steps / (time.perf_counter() - t0)
     # in 2D the last forward pass (timing slot 4) is the y-pass with the
     # Runge-Kutta stage
     names = list(KINDS)
@@ -382,9 +401,11 @@ def bench_ns2d(args, torch, _lib, copy_gbps):
         "step_roofline": {"B_alg_GB_per_step": balg / 1e9,
                           "achieved_GBps": balg * steps_per_s / 1e9,
                           "frac": balg * steps_per_s / 1e9 / PEAK_HBM_GBPS},
+        "hip_graph": {"steps_per_s": graph_steps_per_s,
+                      "step_roofline_frac": balg * graph_steps_per_s / 1e9 / PEAK_HBM_GBPS},
         "kernels": kernels,
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def bench_strat(args, torch, _lib, copy_gbps):
@@ -484,7 +505,7 @@ def bench_strat(args, torch, _lib, copy_gbps):
                        f"{m}^3 FP64 with the oracle ({t_step:.2f} s/step, scipy.fft "
                        f"workers={cores}), scaled by N log N (x{scale:.1f}) to {n}^3"),
         }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def _cpu_model():
@@ -763,7 +784,7 @@ def bench_ensemble(args, torch, _lib, copy_gbps):
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         with contextlib.redirect_stdout(sys.stderr):
             result["cpu_baseline"] = cpu_baseline_ensemble(runs)
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def bench_closure(args, torch, _lib):
@@ -776,7 +797,7 @@ def bench_closure(args, torch, _lib):
         print(f"[closure] {msg}", file=sys.stderr, flush=True)
 
     def new_dns(seed):
-        Cls, p = CL.ns2d_params(nd, nu)
+        Cls, p = CL.ns2d_params(nd, nu, graph=True)
         np.random.seed(seed)
         with contextlib.redirect_stdout(sys.stderr):
             sim = Cls(p)
@@ -824,7 +845,7 @@ def bench_closure(args, torch, _lib):
     wk = CL.cut_to_les(dns.oper, les_oper, dns.state.state_spect.data[0]).cpu().numpy()
     runs = {}
     for name in ("les_closure", "les_plain"):
-        SimulLES, lp = CL.ns2d_params(nl, nu, les=True)
+        SimulLES, lp = CL.ns2d_params(nl, nu, les=True, graph=True)
         np.random.seed(7)
         with contextlib.redirect_stdout(sys.stderr):
             les = SimulLES(lp)
@@ -922,7 +943,134 @@ def bench_closure(args, torch, _lib):
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
         result["cpu_baseline"] = cpu_baseline_closure(net.folded(sw, s_s), nd, nl, nu)
-    print(json.dumps(result), flush=True)
+    return result
+
+
+def closure_reduced(torch, nd=512, nl=64, nu=2e-4, band=(4, 6), nruns=4, nsnaps=50,
+                    train_steps=1000, t_eval=4.0):
+    """The learned-closure workload at reduced size (the GPU test's case):
+    forced DNS nd^2 -> training pairs on nl^2 -> 6-layer network trained on
+    the device -> held-out DNS against the LES with and without the network,
+    same forcing, every run's steps replayed from a HIP graph.  Returns the
+    DNS / LES wall-time ratio over the evaluation, the spectrum errors and
+    the a priori error of the subgrid term."""
+    from fluidsim_amd import closure as CL
+    from fluidsim_amd.synthetic import ns2d_noise_rot_fft
+
+    t00 = time.perf_counter()
+
+    def new_dns(seed):
+        Cls, p = CL.ns2d_params(nd, nu, nkmin=band[0], nkmax=band[1], graph=True)
+        np.random.seed(seed)
+        sim = Cls(p)
+        sim.state.init_statespect_from(
+            rot_fft=ns2d_noise_rot_fft(nd, nd, seed=seed, kmax=12, velo_max=0.5))
+        return sim
+
+    with contextlib.redirect_stdout(sys.stderr):
+        Cls, pl = CL.ns2d_params(nl, nu, forcing=False)
+        les_oper = Cls(pl).oper
+        ws, ss = [], []
+        for r in range(nruns):
+            dns = new_dns(100 + r)
+            CL.run_until(dns, 3.0)
+            for k in range(nsnaps):
+                CL.run_until(dns, 3.0 + (k + 1) * 0.05)
+                w, sg = CL.subgrid_terms(dns, les_oper)
+                ws.append(w.float())
+                ss.append(sg.float())
+            del dns
+        t_data = time.perf_counter() - t00
+        net = CL.CNNClosure(n_in=1, nlayers=6, seed=0)
+        t0 = time.perf_counter()
+        sw, s_s, l1 = CL.train(net, ws, ss, train_steps // 2, batch=32, lr=1e-3, seed=0, log=100)
+        _, _, l2 = CL.train(net, ws, ss, train_steps - train_steps // 2, batch=32, lr=3e-4,
+                            seed=1, log=100)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        folded = net.folded(sw, s_s)
+        dns = new_dns(999)
+        CL.run_until(dns, 3.0)
+        wk = CL.cut_to_les(dns.oper, les_oper, dns.state.state_spect.data[0]).cpu().numpy()
+        runs = {}
+        for name in ("les_closure", "les_plain"):
+            SimulLES, lp = CL.ns2d_params(nl, nu, nkmin=band[0], nkmax=band[1], les=True,
+                                          graph=True)
+            np.random.seed(7)
+            les = SimulLES(lp)
+            les.state.init_statespect_from(rot_fft=wk)
+            les.time_stepping.t = dns.time_stepping.t
+            if name == "les_closure":
+                les.closure_net = folded
+            runs[name] = les
+        runs["dns"] = dns
+        CL.share_forcing(dns, [runs["les_closure"], runs["les_plain"]])
+        spectra, wall, steps = CL.compare_runs(runs, les_oper, t_eval, 0.2)
+        kmax = nl // 3
+        num = den = 0.0
+        for _ in range(8):
+            CL.run_until(dns, dns.time_stepping.t + 0.1)
+            w, sg = CL.subgrid_terms(dns, les_oper)
+            pred = folded.forward(w.float().reshape(1, nl, nl, 1))[0]
+            num += float(((pred - sg.float()) ** 2).sum())
+            den += float((sg.float() ** 2).sum())
+    return {
+        "workload": (f"ns2d forced (tcrandom k in [{band[0]}, {band[1]}], rate 1), nu_2 = {nu}; "
+                     f"DNS {nd}^2 FP32; LES {nl}^2 FP32 with the 6-layer 5x5 32-channel "
+                     f"network trained on {nruns} DNS runs x {nsnaps} filtered snapshots "
+                     f"({train_steps} Adam steps, batch 32); evaluation over {t_eval} time "
+                     "units from a held-out DNS; steps replayed from HIP graphs"),
+        "speedup_dns_over_les_closure": wall["dns"] / wall["les_closure"],
+        "ms_per_step": {k: wall[k] / max(steps[k], 1) * 1e3 for k in wall},
+        "steps": steps,
+        "spectrum_error": {"shells_1_to_kmax": CL.spectrum_errors(spectra, kmax),
+                           "small_scales": CL.spectrum_errors(spectra, kmax, lo=kmax // 2),
+                           "kmax": kmax},
+        "apriori_subgrid_nmse": num / den,
+        "training_loss": [l1[0][1], l2[-1][1]],
+        "timings_s": {"training_data": t_data, "training": t_train,
+                      "total": time.perf_counter() - t00},
+    }
+
+
+def other_configs(args, torch, _lib, copy_gbps):
+    """Short legs of the other workloads, in the default run so that every
+    config is measured in one driver run: ns3d.strat 1024^3 (cfg 5, FP64
+    oracle check at 128^3), the 512-run ensemble (cfg 3), the closure
+    workload's 2048^2 DNS and the closure at reduced size (cfg 4), and the
+    network's training step.  Each leg's failure is recorded, not raised."""
+    import types
+
+    legs = {}
+
+    def leg(name, fn):
+        print(f"[bench] leg {name}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        try:
+            with contextlib.redirect_stdout(sys.stderr):
+                legs[name] = fn()
+        except Exception as e:  # noqa: BLE001 - recorded in the JSON line
+            legs[name] = {"error": f"{type(e).__name__}: {e}"}
+        legs[name]["leg_wall_s"] = time.perf_counter() - t0
+        gc.collect()
+        torch.cuda.empty_cache()
+
+    def ns(**kw):
+        base = dict(vars(args))
+        base.update(kw)
+        return types.SimpleNamespace(**base)
+
+    leg("strat_1024", lambda: bench_strat(
+        ns(n=1024, steps=10, warmup=2, precision="float32", cpu_n=128, oracle_steps=2,
+           no_cpu_baseline=False), torch, _lib, copy_gbps))
+    leg("ensemble_512_runs", lambda: bench_ensemble(
+        ns(steps=100, warmup=3, no_cpu_baseline=True), torch, _lib, copy_gbps))
+    leg("ns2d_2048_dns", lambda: bench_ns2d(
+        ns(n=2048, steps=50, warmup=5, precision="float32"), torch, _lib, copy_gbps))
+    leg("closure_reduced", lambda: closure_reduced(torch))
+    leg("cnn_train", lambda: bench_cnn_train(ns(steps=20, warmup=3, cnn_batch=32,
+                                                cnn_size=128), torch, _lib))
+    return legs
 
 
 def bench_cnn_train(args, torch, _lib):
@@ -1000,7 +1148,7 @@ def bench_cnn_train(args, torch, _lib):
             "flops_per_call": flops,
         },
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def cpu_baseline_closure(net, nd, nl, nu, reps=2):
@@ -1104,6 +1252,8 @@ def main():
                     help="grid of the FP32 / FP64 legs of the ns3d workload")
     ap.add_argument("--no-cfg2", action="store_true",
                     help="skip the FP32 / FP64 legs at --cfg2-n")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the short legs of the other workloads")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first "
                          "(calibration kernel for the HBM byte counters)")
@@ -1124,16 +1274,20 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n, prec = args.n, (4 if args.precision == "float32" else 8)
     copy_gbps = measured_copy_bandwidth(torch)
+    mode = None
     if args.ensemble:
-        return bench_ensemble(args, torch, _lib, copy_gbps)
-    if args.closure:
-        return bench_closure(args, torch, _lib)
-    if args.cnn_train:
-        return bench_cnn_train(args, torch, _lib)
-    if args.strat:
-        return bench_strat(args, torch, _lib, copy_gbps)
-    if args.ns2d:
-        return bench_ns2d(args, torch, _lib, copy_gbps)
+        mode = bench_ensemble(args, torch, _lib, copy_gbps)
+    elif args.closure:
+        mode = bench_closure(args, torch, _lib)
+    elif args.cnn_train:
+        mode = bench_cnn_train(args, torch, _lib)
+    elif args.strat:
+        mode = bench_strat(args, torch, _lib, copy_gbps)
+    elif args.ns2d:
+        mode = bench_ns2d(args, torch, _lib, copy_gbps)
+    if mode is not None:
+        print(json.dumps(mode), flush=True)
+        return
     if args.copy_mb:
         src = torch.ones(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -1256,6 +1410,8 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(n, args.cpu_n, args.cpu_n // 2)
+    if not args.no_legs:
+        result["other_configs"] = other_configs(args, torch, _lib, copy_gbps)
     result["bench_wall_s"] = time.perf_counter() - T_START
     print(json.dumps(result), flush=True)
 

@@ -326,8 +326,9 @@ def make_les_class():
 
 
 def ns2d_params(n, nu_2, precision="float32", forcing=True, nkmin=8, nkmax=10, rate=1.0,
-                les=False):
-    """Parameters of the forced ns2d runs of the closure workload."""
+                les=False, graph=False):
+    """Parameters of the forced ns2d runs of the closure workload (graph:
+    steps replayed from a HIP graph, params.time_stepping.hip_graph)."""
     from .solvers.ns2d.solver import Simul
 
     cls = make_les_class() if les else Simul
@@ -339,6 +340,7 @@ def ns2d_params(n, nu_2, precision="float32", forcing=True, nkmin=8, nkmax=10, r
     params.time_stepping.USE_CFL = True
     params.time_stepping.cfl_coef = 0.5
     params.time_stepping.USE_T_END = False
+    params.time_stepping.hip_graph = graph
     params.init_fields.type = "in_script"
     params.output.HAS_TO_SAVE = False
     params.output.periods_print.print_stdout = 0

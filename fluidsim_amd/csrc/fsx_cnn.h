@@ -279,14 +279,15 @@ static __global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad
 // 16-byte aligned.  Wave 0 also multiplies dy by a column of ones: the bias
 // gradient.
 //
-// Persistent: workgroup g (of G) takes row blocks g, g + G, ... of its
-// XCD's contiguous share (neighbouring row blocks share halo rows in that
-// XCD's L2) and writes its partials part[g][32][25 CI] and, after all G
-// weight partials, part[G 32 25 CI + 32 g + o]; k_cnn_reduce sums them in a
-// fixed order (deterministic).
+// Persistent: workgroup g (of G) takes a contiguous run of row blocks of
+// its XCD's contiguous share and keeps the x patch rows in a ring of
+// kWgRows + 4 slots, so consecutive row blocks of an image stage kWgRows new
+// x rows instead of kWgRows + 4; it writes its partials part[g][32][25 CI]
+// and, after all G weight partials, part[G 32 25 CI + 32 g + o];
+// k_cnn_reduce sums them in a fixed order (deterministic).
 constexpr int kWgRows = 2;      // output rows per row block
 constexpr int kWgThreads = 320; // 5 waves, one per tap row
-constexpr int kWgGroups = 512;  // persistent workgroups (2 per CU)
+constexpr int kWgGroups = 256;  // persistent workgroups (one per CU; halves the partials)
 
 __host__ __device__ inline int wg_dp(int W) { return odd8(W); }
 __host__ __device__ inline int wg_xp(int W) { return odd8(W + 8); }
@@ -348,15 +349,15 @@ __device__ __forceinline__ void wg_chunk(const u32x4& a, const u32x4& x0, const 
 // of the next chunk issued before the MFMAs of the current one, then, for
 // W % 16 == 8, the last 8 pixels (upper half-wave supplies zeros).
 template <bool BIAS>
-__device__ __forceinline__ void wg_rows(const uint16_t* pa0, const uint16_t* px_0, int W,
-                                        int DROW, int XROW, int g, f32x16 (&acc)[5],
+__device__ __forceinline__ void wg_rows(const uint16_t* pa0, const uint16_t* const (&pxrow)[kWgRows],
+                                        int W, int DROW, int g, f32x16 (&acc)[5],
                                         f32x16& accb) {
   const u32x4 ones = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};
   const int wfull = W & ~15;
 #pragma unroll
   for (int r = 0; r < kWgRows; ++r) {
     const uint16_t* pa = pa0 + r * DROW;
-    const uint16_t* pxr = px_0 + r * XROW;
+    const uint16_t* pxr = pxrow[r];
     u32x4 a = *reinterpret_cast<const u32x4*>(pa);
     u32x4 x0 = *reinterpret_cast<const u32x4*>(pxr);
     u32x4 x1 = *reinterpret_cast<const u32x4*>(pxr + 8);
@@ -417,36 +418,44 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
   const int nyb = H / kWgRows;
   const int nblk = B * nyb;
   const int G = gridDim.x;
-  // XCD-contiguous shares of the row blocks (blocks go round-robin to the
-  // 8 XCDs); plain round-robin when G is not a multiple of 8
-  int first, stride, last;
+  // a contiguous run of row blocks per workgroup, the runs of one XCD's
+  // workgroups contiguous too (blocks go round-robin to the 8 XCDs; the
+  // halo rows between runs are then shared in that XCD's L2)
+  int first, last;
   if (G % 8 == 0) {
     const int gx = G / 8, xcd = blockIdx.x % 8, i = blockIdx.x / 8;
-    const int share = (nblk + 7) / 8;
-    first = xcd * share + i;
-    stride = gx;
-    last = min(nblk, (xcd + 1) * share);
+    const int share = (nblk + 7) / 8, per = (share + gx - 1) / gx;
+    first = xcd * share + i * per;
+    last = min(min(first + per, (xcd + 1) * share), nblk);
   } else {
-    first = blockIdx.x;
-    stride = G;
-    last = nblk;
+    const int per = (nblk + G - 1) / G;
+    first = blockIdx.x * per;
+    last = min(first + per, nblk);
   }
   const int ncg = W / 8 + 1;  // 8-column groups of the x patch
-  // this lane's operand rows (pixels-fast): dy channel m, x channel m
+  // this lane's dy operand row (pixels-fast): channel m
   const uint16_t* pa0 = sd + m * DP + wg_pad(m) + 8 * g;
-  const uint16_t* px_0 = sx + wid * XROW + m * XP + wg_pad(m) + 8 * g;
-  for (int blk = first; blk < last; blk += stride) {
+  const int xoff = m * XP + wg_pad(m) + 8 * g;
+  // x rows live in a ring of kWgRows + 4 slots: image row y (unwrapped,
+  // y >= -2) in slot (y + 6) mod 6; the next row block of the same image
+  // keeps 4 rows and stages 2
+  constexpr int NSLOT = kWgRows + 4;
+  int have_b = -1, have_y0 = 0;
+  for (int blk = first; blk < last; ++blk) {
     const int b = blk / nyb, y0 = (blk - b * nyb) * kWgRows;
+    const int pr0 = (b == have_b && y0 == have_y0 + kWgRows) ? NSLOT - kWgRows : 0;
+    have_b = b;
+    have_y0 = y0;
     __syncthreads();
-    // x patch: item = (patch row, column group, channel vector), channel
-    // vector fastest
-    for (int it = threadIdx.x; it < (kWgRows + 4) * NVI * ncg; it += kWgThreads) {
+    // x patch rows pr0 .. NSLOT - 1: item = (patch row, column group,
+    // channel vector), channel vector fastest
+    for (int it = threadIdx.x; it < (NSLOT - pr0) * NVI * ncg; it += kWgThreads) {
       const int q = it % NVI, rk = it / NVI;
-      const int k = rk % ncg, pr = rk / ncg;
-      int gy = y0 - 2 + pr;
-      gy += gy < 0 ? H : (gy >= H ? -H : 0);
+      const int k = rk % ncg, pr = pr0 + rk / ncg;
+      const int yu = y0 - 2 + pr;  // unwrapped image row
+      const int gy = yu < 0 ? yu + H : (yu >= H ? yu - H : yu);
       const uint16_t* row = x + ((size_t)b * H + gy) * W * CI + 8 * q;
-      uint16_t* dst = sx + pr * XROW + 8 * q * XP + wg_pad(8 * q) + 8 * k;
+      uint16_t* dst = sx + ((yu + NSLOT) % NSLOT) * XROW + 8 * q * XP + wg_pad(8 * q) + 8 * k;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         u32x4 v[4];
@@ -481,10 +490,15 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
       }
     }
     __syncthreads();
+    // output row r, tap row wid: image row y0 + r + wid - 2
+    const uint16_t* pxrow[kWgRows];
+#pragma unroll
+    for (int r = 0; r < kWgRows; ++r)
+      pxrow[r] = sx + ((y0 - 2 + r + wid + NSLOT) % NSLOT) * XROW + xoff;
     if (wid == 0)
-      wg_rows<true>(pa0, px_0, W, DROW, XROW, g, acc, accb);
+      wg_rows<true>(pa0, pxrow, W, DROW, g, acc, accb);
     else
-      wg_rows<false>(pa0, px_0, W, DROW, XROW, g, acc, accb);
+      wg_rows<false>(pa0, pxrow, W, DROW, g, acc, accb);
   }
   // partials: n = (5 w + dx) CI + c, lanes m < CI
   float* dst = part + (size_t)blockIdx.x * 32 * N;

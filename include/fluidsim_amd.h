@@ -433,7 +433,7 @@ int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, 
  *   hipMalloc, no library allocation), for callers that cannot allocate:
  *   code under HIP graph capture, tight training loops, harnesses that
  *   forbid allocation.  No counterpart in the reference.  The array holds
- *   the largest partials buffer (512 x (32 x 25 x 32 + 32) floats, 52.5 MB
+ *   the largest partials buffer (256 x (32 x 25 x 32 + 32) floats, 26.2 MB
  *   per device, held until the process ends); sizes beyond it are refused.
  *   Every call on a device shares that device's array: calls that overlap on
  *   one device (other streams, other threads) race. */

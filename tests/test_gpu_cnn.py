@@ -103,7 +103,11 @@ def test_input_gradient_by_flip(ci, co, co_pad):
 @pytest.mark.parametrize("ci,co_in,shape", [(8, 32, (2, 8, 32)), (32, 32, (2, 8, 32)),
                                             (32, 8, (3, 16, 64)), (8, 8, (1, 4, 16)),
                                             (32, 32, (2, 4, 24)), (8, 32, (1, 8, 40)),
-                                            (32, 32, (8, 128, 128))])
+                                            (32, 32, (8, 128, 128)),
+                                            # more row blocks than workgroups: each
+                                            # workgroup keeps its x rows in the ring
+                                            (32, 32, (16, 128, 64)), (8, 32, (20, 64, 32)),
+                                            (32, 8, (9, 128, 40))])
 def test_weight_gradient(ci, co_in, shape):
     rng = np.random.default_rng(3)
     x = rounded(rng.standard_normal(shape + (ci,)))
