@@ -797,7 +797,7 @@ def bench_closure(args, torch, _lib):
         print(f"[closure] {msg}", file=sys.stderr, flush=True)
 
     def new_dns(seed):
-        Cls, p = CL.ns2d_params(nd, nu, graph=True)
+        Cls, p = CL.ns2d_params(nd, nu)
         np.random.seed(seed)
         with contextlib.redirect_stdout(sys.stderr):
             sim = Cls(p)
@@ -845,7 +845,7 @@ def bench_closure(args, torch, _lib):
     wk = CL.cut_to_les(dns.oper, les_oper, dns.state.state_spect.data[0]).cpu().numpy()
     runs = {}
     for name in ("les_closure", "les_plain"):
-        SimulLES, lp = CL.ns2d_params(nl, nu, les=True, graph=True)
+        SimulLES, lp = CL.ns2d_params(nl, nu, les=True)
         np.random.seed(7)
         with contextlib.redirect_stdout(sys.stderr):
             les = SimulLES(lp)
@@ -951,7 +951,7 @@ def closure_reduced(torch, nd=512, nl=64, nu=2e-4, band=(4, 6), nruns=4, nsnaps=
     """The learned-closure workload at reduced size (the GPU test's case):
     forced DNS nd^2 -> training pairs on nl^2 -> 6-layer network trained on
     the device -> held-out DNS against the LES with and without the network,
-    same forcing, every run's steps replayed from a HIP graph.  Returns the
+    same forcing.  Returns the
     DNS / LES wall-time ratio over the evaluation, the spectrum errors and
     the a priori error of the subgrid term."""
     from fluidsim_amd import closure as CL
@@ -960,7 +960,7 @@ def closure_reduced(torch, nd=512, nl=64, nu=2e-4, band=(4, 6), nruns=4, nsnaps=
     t00 = time.perf_counter()
 
     def new_dns(seed):
-        Cls, p = CL.ns2d_params(nd, nu, nkmin=band[0], nkmax=band[1], graph=True)
+        Cls, p = CL.ns2d_params(nd, nu, nkmin=band[0], nkmax=band[1])
         np.random.seed(seed)
         sim = Cls(p)
         sim.state.init_statespect_from(
@@ -994,8 +994,7 @@ def closure_reduced(torch, nd=512, nl=64, nu=2e-4, band=(4, 6), nruns=4, nsnaps=
         wk = CL.cut_to_les(dns.oper, les_oper, dns.state.state_spect.data[0]).cpu().numpy()
         runs = {}
         for name in ("les_closure", "les_plain"):
-            SimulLES, lp = CL.ns2d_params(nl, nu, nkmin=band[0], nkmax=band[1], les=True,
-                                          graph=True)
+            SimulLES, lp = CL.ns2d_params(nl, nu, nkmin=band[0], nkmax=band[1], les=True)
             np.random.seed(7)
             les = SimulLES(lp)
             les.state.init_statespect_from(rot_fft=wk)
@@ -1019,7 +1018,7 @@ def closure_reduced(torch, nd=512, nl=64, nu=2e-4, band=(4, 6), nruns=4, nsnaps=
                      f"DNS {nd}^2 FP32; LES {nl}^2 FP32 with the 6-layer 5x5 32-channel "
                      f"network trained on {nruns} DNS runs x {nsnaps} filtered snapshots "
                      f"({train_steps} Adam steps, batch 32); evaluation over {t_eval} time "
-                     "units from a held-out DNS; steps replayed from HIP graphs"),
+                     "units from a held-out DNS"),
         "speedup_dns_over_les_closure": wall["dns"] / wall["les_closure"],
         "ms_per_step": {k: wall[k] / max(steps[k], 1) * 1e3 for k in wall},
         "steps": steps,

@@ -101,6 +101,7 @@ SIGNATURES = {
     "fsx_plan_create_batch": [P, c_int, c_int, c_double, c_double, c_int, P, c_int],
     "fsx_batch_set_params": [P, ctypes.POINTER(c_double), ctypes.POINTER(c_double)],
     "fsx_batch_set_row": [P, c_int],
+    "fsx_plan_set_tiles": [P, c_int, c_int, c_int],
     "fsx_batch_set_lines": [P, c_int],
     "fsx_ns2d_batch_rk_stage": [P, c_int, c_int, P, P, P, c_double, P],
     "fsx_graph_begin": [P],

@@ -122,6 +122,10 @@ struct fsx_plan_s {
   double* d_bpar;
   int blines;
   int brow;  // row pass of the batch (fsx_batch_set_row)
+  // 2D y-pass tiles of a single plan (fsx_plan_set_tiles): lines per tile of
+  // the inverse and forward y-passes (0: the launcher's choice) and the ns2d
+  // inverse with one output field per workgroup
+  int ylines_inv, ylines_fwd, ysplit;
 };
 
 namespace {
@@ -737,7 +741,14 @@ int ns2d_front(fsx_plan_s* p, const void* rot, double beta, int want_maxima,
   op.dkx = (T)p->dkx;
   op.dky = (T)p->dky;
   note(K_YINV, (double)g.mx * (g.ym.nk() + 4.0 * p->ny) * sizeof(cpx<T>));
-  int rc = launch_strided<T, +1>(op, p->ny, 1, g.mx, 4, p->tw[1], s, true);
+  int rc;
+  if (p->ysplit) {
+    OpYInv2NSOne<T> op1;
+    static_cast<OpYInv2NS<T>&>(op1) = op;
+    rc = launch_strided<T, +1>(op1, p->ny, 1, g.mx, 4, p->tw[1], s, true, p->ylines_inv);
+  } else {
+    rc = launch_strided<T, +1>(op, p->ny, 1, g.mx, 4, p->tw[1], s, true, p->ylines_inv);
+  }
   if (rc) return rc;
   unsigned long long* mxs =
       want_maxima ? reinterpret_cast<unsigned long long*>(p->d_scal) : nullptr;
@@ -1004,7 +1015,7 @@ int do_ns2d_rk(fsx_plan_s* p, int scheme, int stage, void* S, void* tmp,
   op.rk.ext[0] = reinterpret_cast<const cpx<T>*>(ext);
   note(K_ZFWD, (double)g.mx * p->ny * sizeof(cpx<T>) +
                    rk_bytes(p, scheme, stage, 1, sizeof(cpx<T>)));
-  return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s, true);
+  return launch_strided<T, -1>(op, p->ny, 1, g.mx, 1, p->tw[1], s, true, p->ylines_fwd);
 }
 
 // Batched ns2d Runge-Kutta stage over the nb runs of a batch plan: the

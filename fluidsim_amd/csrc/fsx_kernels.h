@@ -556,6 +556,44 @@ struct OpYInv2NS {
   }
 };
 
+// OpYInv2NS with one output field per workgroup (blockIdx.y): four times
+// the workgroups, a quarter of the serial work each, the input read once per
+// output (long transforms with few lines, such as ns2d at 2048^2).
+template <typename T>
+struct OpYInv2NSOne : OpYInv2NS<T> {
+  static constexpr bool kFieldPerBlock = true;
+  static constexpr int kLineMul = 1;
+
+  template <int DIR, int EPT, int kT, bool GEN>
+  __device__ __forceinline__ void run(cpx<T>* buf, const cpx<T>* tw, int o, int line0) const {
+    const StridedDesc<T>& d = this->d;
+    const int nlines = d.nlines;
+    const int c = blockIdx.y;
+    const cpx<T>* in = this->in + o * this->ips;
+    cpx<T>* out = this->out + o * this->ops + c * this->ofs;
+    const int ny = this->ny, nkx = this->nkx, ld2 = this->ld2;
+    const T dkx = this->dkx, dky = this->dky;
+    tile_fft<T, DIR, EPT, kT, GEN>(
+        buf, tw, d, d.logL,
+        [&](int, int l, int p) {
+          const int line = line0 + l;
+          cpx<T> r = zero<T>();
+          if (line < nlines && this->ym.compact(p) >= 0) r = in[p * nkx + line];
+          const T kx = T(line) * dkx, ky = T(ksigned(p, ny)) * dky;
+          T k2 = kx * kx + ky * ky;
+          if (k2 == T(0)) k2 = T(1e-14);
+          const T ik2 = T(1) / k2;
+          // the multipliers of OpYInv2NS's outputs 0..3
+          const T f = c == 0 ? ky * ik2 : (c == 1 ? -kx * ik2 : (c == 2 ? kx : ky));
+          return mul_i(scale(r, f));
+        },
+        [&](int, int l, int p, cpx<T> v) {
+          const int line = line0 + l;
+          if (line < nlines) out[p * ld2 + line] = v;
+        });
+  }
+};
+
 // ns3d z-pass, forward, with the Runge-Kutta stage folded in: the three
 // tendency components are transformed one after the other (the first two
 // kept in registers), then each kept mode is projected, masked and advanced

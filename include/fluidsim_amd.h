@@ -366,6 +366,12 @@ int fsx_plan_create_batch(fsx_plan* out, int nx, int ny, double Lx, double Ly,
                           int nbatch);
 int fsx_batch_set_params(fsx_plan p, const double* dt, const double* visc4);
 int fsx_batch_set_lines(fsx_plan p, int lines);
+/* fsx_plan_set_tiles (no reference counterpart: tile choices of the 2D
+ * y-passes of a single 2D plan, set by an autotuner): lines per tile of the
+ * inverse and of the forward y-passes (powers of two; 0 = the launcher's
+ * choice) and, with inv_split, the ns2d inverse y-pass with one output
+ * field (u, v, d_x rot, d_y rot) per workgroup. */
+int fsx_plan_set_tiles(fsx_plan p, int inv_lines, int fwd_lines, int inv_split);
 /* fsx_batch_set_row: row pass of the batched stages, a choice of the
  * ensemble autotuner: 0 register rows with radix-8 stages (8 points per
  * lane in FP32; default), 1 register rows with radix-4 stages (FP32), 2 the

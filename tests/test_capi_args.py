@@ -46,6 +46,7 @@ CASES = [
     ("fsx_cnn_wgrad_workspace", (1, 8, 16, ctypes.byref(ctypes.c_longlong())), "bad argument"),
     ("fsx_cnn_conv_wgrad", (P, 16, P, 32, 1, 8, 32, P, P, P, 1 << 20, None), "channels"),
     ("fsx_cnn_conv_wgrad", (P, 32, P, 32, 1, 6, 32, P, P, P, 1 << 20, None), "multiple of 4"),
+    ("fsx_plan_set_tiles", (None, 0, 0, 0), "bad argument"),
     ("fsx_cnn_conv_wgrad_noalloc", (P, 16, P, 32, 1, 8, 32, P, P, None), "channels"),
     ("fsx_cnn_conv_wgrad_noalloc", (P, 32, P, 32, 1, 6, 32, P, P, None), "multiple of 4"),
     ("fsx_cnn_conv_wgrad_noalloc", (P, 32, P, 32, 1, 8, 36, P, P, None), "multiple of 4"),

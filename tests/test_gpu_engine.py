@@ -103,6 +103,25 @@ def test_ns2d_tendency(prec, n):
     assert sc[1] == pytest.approx(np.abs(ph["uy"]).max(), rel=1e-12 if prec == 8 else 1e-5)
 
 
+@pytest.mark.parametrize("n", [256, 2048])
+def test_ns2d_tile_settings_bit_identical(n):
+    """fsx_plan_set_tiles changes only the tiling of the y-passes: the
+    inverse with one output field per workgroup or all four, any lines per
+    tile, give the same tendency bit for bit (FP32)."""
+    print(f"case ns2d tile settings n={n}", flush=True)
+    oper, sim = _ns2d_case(n)
+    plan = make_plan(oper, 4, 4)
+    rot = to_dev_complex(sim.state_spect[0], 4)
+    outs = []
+    for split, inv, fwd in ((0, 0, 0), (1, 0, 0), (1, 2, 1), (0, 8, 4)):
+        _lib.call("fsx_plan_set_tiles", plan.handle, inv, fwd, split)
+        out = zeros_complex(oper.shapeK, 4)
+        _lib.call("fsx_ns2d_tendency", plan.handle, _lib.dptr(rot), _lib.dptr(out), 0.0, 0, S())
+        outs.append(host(out))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
 def _ns3d_case(shape, seed=0):
     nz, ny, nx = shape
     oper = Oper3D(nx, ny, nz)
