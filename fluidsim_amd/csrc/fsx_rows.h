@@ -206,6 +206,7 @@ struct XRDesc {
 template <typename T>
 struct XRNS3D {
   static constexpr int kIn = 6, kOut = 3, kMax = 3;
+  static constexpr bool kTwoPhase = false;
   T beta;  // unused
   template <int P>
   __device__ __forceinline__ void apply(cpx<T> (&f)[kIn][P], T (&m)[kMax],
@@ -226,23 +227,26 @@ struct XRNS3D {
   }
 };
 
+// ns3d.strat: (v, omega, b) -> (v x omega, v b) in two phases, so that at
+// most six fields are live per lane (seven at once spill at two waves per
+// SIMD): phase A transforms v and omega and writes v x omega over omega
+// (outputs 0..2), phase B transforms b and writes v b over v (outputs 3..5).
 template <typename T>
 struct XRStrat {
   static constexpr int kIn = 7, kOut = 6, kMax = 3;
+  static constexpr bool kTwoPhase = true;
+  static constexpr int kInA = 6, kOutA = 3, kOutAReg = 3;
   T beta;  // unused
   template <int P>
-  __device__ __forceinline__ void apply(cpx<T> (&f)[kIn][P], T (&m)[kMax],
-                                        bool valid) const {
+  __device__ __forceinline__ void apply_a(cpx<T> (&f)[kInA][P], T (&m)[kMax],
+                                          bool valid) const {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       const cpx<T> vx = f[0][i], vy = f[1][i], vz = f[2][i];
-      const cpx<T> wx = f[3][i], wy = f[4][i], wz = f[5][i], b = f[6][i];
-      f[0][i] = mk<T>(vy.x * wz.x - vz.x * wy.x, vy.y * wz.y - vz.y * wy.y);
-      f[1][i] = mk<T>(vz.x * wx.x - vx.x * wz.x, vz.y * wx.y - vx.y * wz.y);
-      f[2][i] = mk<T>(vx.x * wy.x - vy.x * wx.x, vx.y * wy.y - vy.y * wx.y);
-      f[3][i] = mk<T>(vx.x * b.x, vx.y * b.y);
-      f[4][i] = mk<T>(vy.x * b.x, vy.y * b.y);
-      f[5][i] = mk<T>(vz.x * b.x, vz.y * b.y);
+      const cpx<T> wx = f[3][i], wy = f[4][i], wz = f[5][i];
+      f[3][i] = mk<T>(vy.x * wz.x - vz.x * wy.x, vy.y * wz.y - vz.y * wy.y);
+      f[4][i] = mk<T>(vz.x * wx.x - vx.x * wz.x, vz.y * wx.y - vx.y * wz.y);
+      f[5][i] = mk<T>(vx.x * wy.x - vy.x * wx.x, vx.y * wy.y - vy.y * wx.y);
       if (valid) {
         m[0] = fmax(m[0], fmax(fabs(vx.x), fabs(vx.y)));
         m[1] = fmax(m[1], fmax(fabs(vy.x), fabs(vy.y)));
@@ -250,12 +254,20 @@ struct XRStrat {
       }
     }
   }
+  template <int P>
+  __device__ __forceinline__ void apply_b(cpx<T> (&f)[kInA][P], const cpx<T> (&b)[P]) const {
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) f[c][i] = mk<T>(f[c][i].x * b[i].x, f[c][i].y * b[i].y);
+  }
 };
 
 // ns2d: (ux, uy, d_x rot, d_y rot) -> -ux d_x rot - uy (d_y rot + beta)
 template <typename T>
 struct XRNS2D {
   static constexpr int kIn = 4, kOut = 1, kMax = 2;
+  static constexpr bool kTwoPhase = false;
   T beta;
   template <int P>
   __device__ __forceinline__ void apply(cpx<T> (&f)[kIn][P], T (&m)[kMax],
@@ -355,10 +367,9 @@ void k_xrow(XRDesc<T> d, Op op) {
     const bool valid = rw < nv;
     const unsigned in_bytes = (unsigned)nv * d.ld_in * cs;
     const unsigned out_bytes = (unsigned)nv * d.ld_out * cs;
-    cpx<T> f[Op::kIn][P];
-    // c2r pre-processing into registers, then the inverse transforms
-#pragma unroll
-    for (int fi = 0; fi < Op::kIn; ++fi) {
+    // c2r pre-processing of input field fi into registers, then the inverse
+    // transform
+    auto load_inv = [&](int fi, cpx<T>(&z)[P]) {
       const __amdgpu_buffer_rsrc_t rs =
           buf_rsrc(d.in + fi * d.ifs + (long long)wrow0 * d.ld_in, in_bytes);
 #pragma unroll
@@ -371,17 +382,16 @@ void k_xrow(XRDesc<T> d, Op op) {
         }
         const cpx<T> w = tw.pair[q];
         const cpx<T> dk = Xk - conj(Xm);
-        f[fi][q] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
+        z[q] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
       }
-      row_fft<T, +1, LOGM, LOGP>(f[fi], xb, l, tw);
-    }
-    op.template apply<P>(f, mloc, valid);
-    // forward transforms of the products and r2c post-processing
-#pragma unroll
-    for (int fo = 0; fo < Op::kOut; ++fo) {
+      row_fft<T, +1, LOGM, LOGP>(z, xb, l, tw);
+    };
+    // forward transform of product registers z and r2c post-processing into
+    // output field fo
+    auto fwd_store = [&](int fo, const cpx<T>(&z)[P]) {
       cpx<T> v[P];
 #pragma unroll
-      for (int r = 0; r < P; ++r) v[r] = f[fo][PL::ireg(r)];
+      for (int r = 0; r < P; ++r) v[r] = z[PL::ireg(r)];
       row_fft<T, -1, LOGM, LOGP>(v, xb, l, tw);
       // mirror exchange: Z[(M - k) mod M] for k = l + G qpos(i)
       wave_sync();
@@ -402,6 +412,29 @@ void k_xrow(XRDesc<T> d, Op op) {
         buf_store<T>(rd, offo[q], scale(E + Pk, d.scale));
         if (q == 0) buf_store<T>(rd, offn, scale(conj(E - Pk), d.scale));
       }
+    };
+    if constexpr (Op::kTwoPhase) {
+      // phase A from inputs 0..kInA-1, phase B adds input kInA: fewer
+      // fields live at once (see the op)
+      cpx<T> f[Op::kInA][P];
+#pragma unroll
+      for (int fi = 0; fi < Op::kInA; ++fi) load_inv(fi, f[fi]);
+      op.template apply_a<P>(f, mloc, valid);
+#pragma unroll
+      for (int fo = 0; fo < Op::kOutA; ++fo) fwd_store(fo, f[Op::kOutAReg + fo]);
+      cpx<T> b[P];
+      load_inv(Op::kInA, b);
+      op.template apply_b<P>(f, b);
+#pragma unroll
+      for (int fo = Op::kOutA; fo < Op::kOut; ++fo) fwd_store(fo, f[fo - Op::kOutA]);
+    } else {
+      cpx<T> f[Op::kIn][P];
+#pragma unroll
+      for (int fi = 0; fi < Op::kIn; ++fi) load_inv(fi, f[fi]);
+      op.template apply<P>(f, mloc, valid);
+      // forward transforms of the products and r2c post-processing
+#pragma unroll
+      for (int fo = 0; fo < Op::kOut; ++fo) fwd_store(fo, f[fo]);
     }
   }
 
