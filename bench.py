@@ -519,7 +519,7 @@ def _cpu_model():
     return "unknown"
 
 
-def _oracle_step_time(n, workers, steps=2):
+def _oracle_step_time(n, workers, steps=3):
     """Median wall time of `steps` RK4 steps of the ns3d workload at n^3 with
     the FP64 oracle (scipy.fft workers, numpy elementwise), after one
     warm-up step (fluidsim-bench style: fixed dt, no output)."""
