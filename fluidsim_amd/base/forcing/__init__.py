@@ -64,8 +64,10 @@ class ForcingBase:
         self.type_forcing = params.forcing.type
         dict_classes = _maker_classes(sim.info_solver)
         if self.type_forcing not in dict_classes:
+            known = ", ".join(sorted(dict_classes)) or "none registered"
             raise ValueError(
-                "Wrong value for params.forcing.type: " + str(self.type_forcing)
+                f"params.forcing.type = {self.type_forcing!r} names no forcing of this "
+                f"solver (known: {known})"
             )
         self.sim = sim
         self.forcing_maker = dict_classes[self.type_forcing](sim)

@@ -54,18 +54,19 @@ class TurbModel:
 
     def __init__(self, sim):
         self.type_model = sim.params.turb_model.type
-        dict_classes = _model_classes(sim.info_solver)
-        try:
-            cls_model = dict_classes[self.type_model]
-        except KeyError:
+        models = _model_classes(sim.info_solver)
+        if self.type_model not in models:
+            known = ", ".join(sorted(models)) or "none registered"
             raise ValueError(
-                f"Wrong value ('{self.type_model}') of params.turb_model.type. "
-                f"It should be in {dict_classes.keys()}"
+                f"params.turb_model.type = {self.type_model!r} names no turbulence "
+                f"model of this solver (known: {known})"
             )
-        self._model = cls_model(sim)
+        self._model = models[self.type_model](sim)
 
     def get_forcing(self, **kwargs):
-        return self._model.get_forcing(**kwargs)
+        """Model tendency of the current state (the specific model's)."""
+        model = self._model
+        return model.get_forcing(**kwargs)
 
 
 class SpecificTurbModel(SimulExtender):
@@ -91,11 +92,13 @@ class SpecificTurbModel(SimulExtender):
 
 
 class SpecificTurbModelSpectral(SpecificTurbModel):
+    """A model whose tendency is a set of spectral arrays shaped like the
+    state (zero until the first get_forcing)."""
+
     def __init__(self, sim):
         self.sim = sim
-        self.forcing_fft = SetOfVariables(
-            like=sim.state.state_spect, info="forcing_fft", value=0.0
-        )
+        like = sim.state.state_spect
+        self.forcing_fft = SetOfVariables(like=like, info="forcing_fft", value=0.0)
 
 
 class SmagorinskyModel(SpecificTurbModelSpectral):
