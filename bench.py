@@ -136,8 +136,15 @@ def make_strat_sim(n, precision, seed=0, fixed_dt=None):
     np.random.seed(seed)  # forcing seeds and renewals
     sim = Simul(params)
     cdt = np.complex64 if precision == "float32" else np.complex128
-    vv = ns3d_spectrum_vv_fft(n, seed=seed, dtype=cdt)
+    if n > 512:
+        # the device generator (the host one needs minutes at 1024^3)
+        from fluidsim_amd.synthetic import ns3d_spectrum_vv_device
+
+        vv = ns3d_spectrum_vv_device(sim.oper, seed=seed)
+    else:
+        vv = ns3d_spectrum_vv_fft(n, seed=seed, dtype=cdt)
     sim.state.init_statespect_from(vx_fft=vv[0], vy_fft=vv[1], vz_fft=vv[2])
+    del vv
     return sim
 
 
@@ -470,7 +477,9 @@ def bench_strat(args, torch, _lib, copy_gbps):
             "workload": (
                 f"ns3d.strat {n}^3 {args.precision} RK4, N = 4, Pr = 1, nu_8 with "
                 "k_max*eta = 1, toroidal tcrandom_anisotropic forcing at k in [2, 4], "
-                "isotropic E(k)~k^4 exp(-2(k/4)^2) velocity (seed 0), b = 0, CFL 0.5"
+                "isotropic E(k)~k^4 exp(-2(k/4)^2) velocity (seed 0, "
+                + ("device generator" if n > 512 else "host generator")
+                + "), b = 0, CFL 0.5"
             ),
             "grid": [n, n, n],
             "parallelism": "single GPU",

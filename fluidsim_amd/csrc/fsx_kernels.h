@@ -39,14 +39,24 @@ struct StridedDesc {
   int rad[kMaxStages];
 };
 
+// Hook of tile_fft run after the first stage (default: nothing).
+struct NoFetch {
+  __device__ __forceinline__ void operator()(int, int, int) const {}
+};
+
 // Full transform of one tile.  src(e, l, p) gives the input at position p
 // of local line l (e = register slot of the first stage, stable across
 // calls); dst(e, l, p, v) receives the output at position p (e = register
-// slot of the last stage).  Ends with a barrier so buf can be reused.
-template <typename T, int DIR, int EPT, int kT, class Src, class Dst>
+// slot of the last stage).  fetch(e, l, p) is called for every first-stage
+// slot once that slot's input has been consumed: a multi-transform pass
+// issues there the global loads of its next transform (same slots), so
+// they are in flight during this transform's LDS stages.  Ends with a
+// barrier so buf can be reused.
+template <typename T, int DIR, int EPT, int kT, class Src, class Dst,
+          class Fetch = NoFetch>
 __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
                                             int logN, int logL, Src&& src,
-                                            Dst&& dst) {
+                                            Dst&& dst, const Fetch& fetch = Fetch()) {
   const int lmask = (1 << logL) - 1;
   if (logN <= 3) {
     // single stage: HBM -> registers -> HBM
@@ -63,6 +73,8 @@ __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
           dftR<T, DIR, R>(v);
 #pragma unroll
           for (int r = 0; r < R; ++r) dst(i * R + r, b, r, v[r]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) fetch(i * R + r, b, r);
         }
       }
     };
@@ -92,6 +104,8 @@ __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
         dft8<T, DIR>(v);
 #pragma unroll
         for (int r = 0; r < 8; ++r) buf[lpad((((j << 3) + r) << logL) + line)] = v[r];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) fetch(i * 8 + r, line, j + (r << lognb));
       }
     }
     __syncthreads();
@@ -138,12 +152,13 @@ __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
 // power-of-two length (tile_fft_p2, registers at both ends).  GEN = true:
 // any length (gen_fft); slot e of both src and dst is the tile element
 // threadIdx.x + e kT (line-fast), and the tile uses two LDS buffers.
-template <typename T, int DIR, int EPT, int kT, bool GEN, class Src, class Dst>
+template <typename T, int DIR, int EPT, int kT, bool GEN, class Src, class Dst,
+          class Fetch = NoFetch>
 __device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
                                          const StridedDesc<T>& d, int logL, Src&& src,
-                                         Dst&& dst) {
+                                         Dst&& dst, const Fetch& fetch = Fetch()) {
   if constexpr (!GEN) {
-    tile_fft_p2<T, DIR, EPT, kT>(buf, tw, d.logN, logL, src, dst);
+    tile_fft_p2<T, DIR, EPT, kT>(buf, tw, d.logN, logL, src, dst, fetch);
   } else {
     const int tot = d.N << logL;
     const int lmask = (1 << logL) - 1;
@@ -154,6 +169,11 @@ __device__ __forceinline__ void tile_fft(cpx<T>* buf, const cpx<T>* tw,
     for (int e = 0; e < EPT; ++e) {
       const int idx = threadIdx.x + e * kT;
       if (idx < tot) A[lpad(idx)] = src(e, idx & lmask, idx >> logL);
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = threadIdx.x + e * kT;
+      if (idx < tot) fetch(e, idx & lmask, idx >> logL);
     }
     __syncthreads();
     const cpx<T>* res = gen_fft<T, DIR, kT>(A, B, lay, tw, d.N, 1 << logL, d.nst, d.rad);
@@ -413,44 +433,42 @@ struct OpZInvCurl {
       if (line < nlines) out[c * ofs + (p * nyk + o) * ld1 + line] = v;
     };
     auto kz_of = [&](int p) { return T(ksigned(p, nz)) * dkz; };
-    // Vz
+    // Each input is loaded once, into sv (vy, then vx) or sVz (b, once Vz
+    // is no longer needed), while the transform before the one that first
+    // uses it runs (tile_fft's fetch hook): only the first load of a tile
+    // waits for HBM.
+    // Vz; vy fetched
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
         [&](int e, int l, int p, cpx<T> v) {
           sVz[e] = v;
           st(2, l, p, v);
-        });
+        },
+        [&](int e, int l, int p) { sv[e] = ld(1, l, p); });
     // omega_x = i ky Vz - Dz[vy]
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL,
-        [&](int e, int l, int p) {
-          const cpx<T> a = ld(1, l, p);
-          sv[e] = a;
-          return mul_i(scale(a, kz_of(p)));
-        },
+        [&](int e, int, int p) { return mul_i(scale(sv[e], kz_of(p))); },
         [&](int e, int l, int p, cpx<T> v) {
           st(3, l, p, mul_i(scale(sVz[e], ky)) - v);
         });
-    // Vy
+    // Vy; vx fetched over vy once the first stage has read it
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL, [&](int e, int, int) { return sv[e]; },
         [&](int e, int l, int p, cpx<T> v) {
           sVy[e] = v;
           st(1, l, p, v);
-        });
+        },
+        [&](int e, int l, int p) { sv[e] = ld(0, l, p); });
     // omega_y = Dz[vx] - i kx Vz
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL,
-        [&](int e, int l, int p) {
-          const cpx<T> a = ld(0, l, p);
-          sv[e] = a;
-          return mul_i(scale(a, kz_of(p)));
-        },
+        [&](int e, int, int p) { return mul_i(scale(sv[e], kz_of(p))); },
         [&](int e, int l, int p, cpx<T> v) {
           const T kx = T(line0 + l) * dkx;
           st(4, l, p, v - mul_i(scale(sVz[e], kx)));
         });
-    // Vx and omega_z = i kx Vy - i ky Vx (+ f)
+    // Vx and omega_z = i kx Vy - i ky Vx (+ f); b fetched into sVz
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL, [&](int e, int, int) { return sv[e]; },
         [&](int e, int l, int p, cpx<T> v) {
@@ -459,10 +477,13 @@ struct OpZInvCurl {
           cpx<T> w = mul_i(scale(sVy[e], kx) - scale(v, ky));
           if (has_f && iy == 0 && line0 + l == 0) w.x += fcor;
           st(5, l, p, w);
+        },
+        [&](int e, int l, int p) {
+          if (with_b) sVz[e] = ld(3, l, p);
         });
     if (with_b)
       tile_fft<T, DIR, EPT, kT, GEN>(
-          buf, tw, d, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
+          buf, tw, d, d.logL, [&](int e, int, int) { return sVz[e]; },
           [&](int, int l, int p, cpx<T> v) { st(6, l, p, v); });
   }
 };
@@ -623,14 +644,18 @@ struct OpZFwdRK3 {
       if (line >= nlines) return zero<T>();
       return in[c * ifs + (p * nyk + o) * ld1 + line];
     };
+    // input c + 1 is fetched into s(c+1) while transform c runs; the
+    // stash's first stage reads it before its last stage stores the output
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
-        [&](int e, int, int, cpx<T> v) { s0[e] = v; });
+        [&](int e, int, int, cpx<T> v) { s0[e] = v; },
+        [&](int e, int l, int p) { s1[e] = ld(1, l, p); });
     tile_fft<T, DIR, EPT, kT, GEN>(
-        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
-        [&](int e, int, int, cpx<T> v) { s1[e] = v; });
+        buf, tw, d, d.logL, [&](int e, int, int) { return s1[e]; },
+        [&](int e, int, int, cpx<T> v) { s1[e] = v; },
+        [&](int e, int l, int p) { s2[e] = ld(2, l, p); });
     tile_fft<T, DIR, EPT, kT, GEN>(
-        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
+        buf, tw, d, d.logL, [&](int e, int, int) { return s2[e]; },
         [&](int e, int, int, cpx<T> v) { s2[e] = v; });
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -680,18 +705,23 @@ struct OpZFwdRKStrat {
       if (line >= nlines) return zero<T>();
       return in[c * ifs + (p * nyk + o) * ld1 + line];
     };
+    // inputs 1-3 are fetched into s1, s2, sb while the transform before
+    // runs (see OpZFwdRK3)
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL, [&](int, int l, int p) { return ld(0, l, p); },
-        [&](int e, int, int, cpx<T> v) { s0[e] = v; });
+        [&](int e, int, int, cpx<T> v) { s0[e] = v; },
+        [&](int e, int l, int p) { s1[e] = ld(1, l, p); });
     tile_fft<T, DIR, EPT, kT, GEN>(
-        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(1, l, p); },
-        [&](int e, int, int, cpx<T> v) { s1[e] = v; });
+        buf, tw, d, d.logL, [&](int e, int, int) { return s1[e]; },
+        [&](int e, int, int, cpx<T> v) { s1[e] = v; },
+        [&](int e, int l, int p) { s2[e] = ld(2, l, p); });
     tile_fft<T, DIR, EPT, kT, GEN>(
-        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(2, l, p); },
-        [&](int e, int, int, cpx<T> v) { s2[e] = v; });
+        buf, tw, d, d.logL, [&](int e, int, int) { return s2[e]; },
+        [&](int e, int, int, cpx<T> v) { s2[e] = v; },
+        [&](int e, int l, int p) { sb[e] = ld(3, l, p); });
     // sb = kx G_x + ky G_y + kz G_z  (N_b = -i sb - N^2 v_z)
     tile_fft<T, DIR, EPT, kT, GEN>(
-        buf, tw, d, d.logL, [&](int, int l, int p) { return ld(3, l, p); },
+        buf, tw, d, d.logL, [&](int e, int, int) { return sb[e]; },
         [&](int e, int l, int, cpx<T> v) { sb[e] = scale(v, T(line0 + l) * dkx); });
     tile_fft<T, DIR, EPT, kT, GEN>(
         buf, tw, d, d.logL, [&](int, int l, int p) { return ld(4, l, p); },
