@@ -358,24 +358,51 @@ void k_xrow(XRDesc<T> d, Op op) {
 #pragma unroll
   for (int i = 0; i < Op::kMax; ++i) mloc[i] = T(0);
 
-  for (int row0 = blockIdx.x * RPB; row0 < d.nrows; row0 += gridDim.x * RPB) {
-    const int wrow0 = row0 + wave * RPW;
-    int nv = d.nrows - wrow0;
+  // prefetch across row groups where the 32 (FP32) extra live registers
+  // fit without spilling (measured on the instantiations: not FP64, not
+  // the six-field ns3d rows of length >= 1024)
+  constexpr bool kNext = sizeof(T) == 4 && (LOGM < 9 || Op::kIn < 6 || Op::kTwoPhase);
+  const int rstride = gridDim.x * RPB;
+  // valid rows of the wave in the row group starting at row0 (uniform: kept
+  // in scalar registers so the resources need no waterfall)
+  auto wave_rows = [&](int row0) {
+    int nv = d.nrows - (row0 + wave * RPW);
     nv = nv < 0 ? 0 : (nv > RPW ? RPW : nv);
-    // (uniform: kept in scalar registers so the resources need no waterfall)
-    nv = __builtin_amdgcn_readfirstlane(nv);
+    return __builtin_amdgcn_readfirstlane(nv);
+  };
+  // Input fields are loaded into (xk, xm) one transform ahead: the loads of
+  // field fi + 1 are issued after the c2r pre-processing of field fi and
+  // before its inverse transform, and those of field 0 of the next row
+  // group before the last forward transform, so they are in flight during
+  // the transforms' exchanges.
+  cpx<T> xk[P], xm[P];
+  auto fetch_at = [&](int row0, int fi) {
+    const __amdgpu_buffer_rsrc_t rs =
+        buf_rsrc(d.in + fi * d.ifs + (long long)(row0 + wave * RPW) * d.ld_in,
+                 (unsigned)wave_rows(row0) * d.ld_in * cs);
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      xk[q] = buf_load<T>(rs, offk[q]);
+      xm[q] = buf_load<T>(rs, offm[q]);
+    }
+  };
+  if (kNext && blockIdx.x * RPB < d.nrows) fetch_at(blockIdx.x * RPB, 0);
+
+  for (int row0 = blockIdx.x * RPB; row0 < d.nrows; row0 += rstride) {
+    const int wrow0 = row0 + wave * RPW;
+    const int nv = wave_rows(row0);
     const bool valid = rw < nv;
-    const unsigned in_bytes = (unsigned)nv * d.ld_in * cs;
     const unsigned out_bytes = (unsigned)nv * d.ld_out * cs;
-    // c2r pre-processing of input field fi into registers, then the inverse
-    // transform
-    auto load_inv = [&](int fi, cpx<T>(&z)[P]) {
-      const __amdgpu_buffer_rsrc_t rs =
-          buf_rsrc(d.in + fi * d.ifs + (long long)wrow0 * d.ld_in, in_bytes);
+    const bool more = kNext && row0 + rstride < d.nrows;
+    if (!kNext) fetch_at(row0, 0);
+    auto fetch = [&](int fi) { fetch_at(row0, fi); };
+    // c2r pre-processing of the fetched field into registers, the next
+    // field's fetch (next >= 0), then the inverse transform
+    auto load_inv = [&](int next, cpx<T>(&z)[P]) {
 #pragma unroll
       for (int q = 0; q < P; ++q) {
-        cpx<T> Xk = buf_load<T>(rs, offk[q]);
-        cpx<T> Xm = buf_load<T>(rs, offm[q]);
+        cpx<T> Xk = xk[q];
+        cpx<T> Xm = xm[q];
         if (q == 0 && l == 0) {
           Xk.y = T(0);
           Xm.y = T(0);
@@ -384,6 +411,7 @@ void k_xrow(XRDesc<T> d, Op op) {
         const cpx<T> dk = Xk - conj(Xm);
         z[q] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
       }
+      if (next >= 0) fetch(next);
       row_fft<T, +1, LOGM, LOGP>(z, xb, l, tw);
     };
     // forward transform of product registers z and r2c post-processing into
@@ -418,23 +446,33 @@ void k_xrow(XRDesc<T> d, Op op) {
       // fields live at once (see the op)
       cpx<T> f[Op::kInA][P];
 #pragma unroll
-      for (int fi = 0; fi < Op::kInA; ++fi) load_inv(fi, f[fi]);
+      for (int fi = 0; fi < Op::kInA; ++fi) load_inv(fi + 1 < Op::kInA ? fi + 1 : -1, f[fi]);
       op.template apply_a<P>(f, mloc, valid);
 #pragma unroll
-      for (int fo = 0; fo < Op::kOutA; ++fo) fwd_store(fo, f[Op::kOutAReg + fo]);
+      for (int fo = 0; fo < Op::kOutA; ++fo) {
+        // input kInA in flight during phase A's last forward transform
+        if (fo == Op::kOutA - 1) fetch(Op::kInA);
+        fwd_store(fo, f[Op::kOutAReg + fo]);
+      }
       cpx<T> b[P];
-      load_inv(Op::kInA, b);
+      load_inv(-1, b);
       op.template apply_b<P>(f, b);
 #pragma unroll
-      for (int fo = Op::kOutA; fo < Op::kOut; ++fo) fwd_store(fo, f[fo - Op::kOutA]);
+      for (int fo = Op::kOutA; fo < Op::kOut; ++fo) {
+        if (fo == Op::kOut - 1 && more) fetch_at(row0 + rstride, 0);
+        fwd_store(fo, f[fo - Op::kOutA]);
+      }
     } else {
       cpx<T> f[Op::kIn][P];
 #pragma unroll
-      for (int fi = 0; fi < Op::kIn; ++fi) load_inv(fi, f[fi]);
+      for (int fi = 0; fi < Op::kIn; ++fi) load_inv(fi + 1 < Op::kIn ? fi + 1 : -1, f[fi]);
       op.template apply<P>(f, mloc, valid);
       // forward transforms of the products and r2c post-processing
 #pragma unroll
-      for (int fo = 0; fo < Op::kOut; ++fo) fwd_store(fo, f[fo]);
+      for (int fo = 0; fo < Op::kOut; ++fo) {
+        if (fo == Op::kOut - 1 && more) fetch_at(row0 + rstride, 0);
+        fwd_store(fo, f[fo]);
+      }
     }
   }
 
