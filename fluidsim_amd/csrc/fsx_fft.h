@@ -120,10 +120,66 @@ __device__ __forceinline__ void dft8(cpx<T>* v) {
   }
 }
 
+// a exp(DIR 2 pi i m / 16) for a constant m
+template <typename T, int DIR, int M>
+__device__ __forceinline__ cpx<T> w16(cpx<T> a) {
+  constexpr int m = M & 15;
+  if constexpr (m == 0) {
+    return a;
+  } else if constexpr (m == 4) {
+    return rot90<T, DIR>(a);
+  } else {
+    // cos and sin of 2 pi m / 16 from the first-octant values
+    constexpr T c1 = T(0.92387953251128675612818318939678828682);  // cos(pi/8)
+    constexpr T s1 = T(0.38268343236508977172845998403039886676);  // sin(pi/8)
+    constexpr T h = T(0.70710678118654752440084436210484903928);
+    constexpr T ct[16] = {T(1), c1, h, s1, T(0), -s1, -h, -c1, T(-1), -c1, -h, -s1, T(0), s1, h, c1};
+    constexpr T c = ct[m], sn = ct[(m + 12) & 15];  // sin x = cos(x - pi/2)
+    const T wi = DIR < 0 ? -sn : sn;
+    // explicit fused multiply-adds: the rounding is then the same in every
+    // pass that inlines this (no choice left to fp contraction)
+    if constexpr (sizeof(T) == 4)
+      return mk<T>(__builtin_fmaf(a.x, c, -(a.y * wi)), __builtin_fmaf(a.x, wi, a.y * c));
+    else
+      return mk<T>(__builtin_fma(a.x, c, -(a.y * wi)), __builtin_fma(a.x, wi, a.y * c));
+  }
+}
+
+// 16-point DFT as 4 x 4: with n = 4 n1 + n2 and k = k1 + 4 k2,
+// X[k1 + 4 k2] = DFT4_{n2}( W16^{n2 k1} DFT4_{n1}(x[4 n1 + n2])[k1] )[k2].
+template <typename T, int DIR>
+__device__ __forceinline__ void dft16(cpx<T>* v) {
+  cpx<T> y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = v[4 * n1 + n2];
+    dft4<T, DIR>(y[n2]);
+  }
+  y[1][1] = w16<T, DIR, 1>(y[1][1]);
+  y[1][2] = w16<T, DIR, 2>(y[1][2]);
+  y[1][3] = w16<T, DIR, 3>(y[1][3]);
+  y[2][1] = w16<T, DIR, 2>(y[2][1]);
+  y[2][2] = w16<T, DIR, 4>(y[2][2]);
+  y[2][3] = w16<T, DIR, 6>(y[2][3]);
+  y[3][1] = w16<T, DIR, 3>(y[3][1]);
+  y[3][2] = w16<T, DIR, 6>(y[3][2]);
+  y[3][3] = w16<T, DIR, 9>(y[3][3]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cpx<T> b[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<T, DIR>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+  }
+}
+
 template <typename T, int DIR, int R>
 __device__ __forceinline__ void dftR(cpx<T>* v) {
-  static_assert(R == 8 || R == 4 || R == 2, "radix 2, 4 or 8");
-  if constexpr (R == 8)
+  static_assert(R == 16 || R == 8 || R == 4 || R == 2, "radix 2, 4, 8 or 16");
+  if constexpr (R == 16)
+    dft16<T, DIR>(v);
+  else if constexpr (R == 8)
     dft8<T, DIR>(v);
   else if constexpr (R == 4)
     dft4<T, DIR>(v);

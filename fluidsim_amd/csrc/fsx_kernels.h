@@ -52,13 +52,37 @@ struct NoFetch {
 // issues there the global loads of its next transform (same slots), so
 // they are in flight during this transform's LDS stages.  Ends with a
 // barrier so buf can be reused.
+// Stage plan of tile_fft_p2 for a length 2^logN with EPT elements per
+// thread: log2 of the first stage's radix (0: a single stage of radix
+// 2^logN) and of the last stage's; the stages between are radix 8.  With
+// 16 elements per thread a radix-16 first stage (and a radix-16 last one
+// where four factors are left) replaces a radix-8 stage plus a radix-2 or
+// -4 one wherever logN is not a multiple of 3: one LDS round trip and its
+// two barriers fewer per tile (1024 = 16 8 8, 2048 = 16 8 16, 256 = 16 16).
+template <int EPT>
+__device__ __forceinline__ void tile_plan(int logN, int& lfirst, int& llast) {
+  if (logN <= 3 || (EPT >= 16 && logN == 4)) {
+    lfirst = 0;
+    llast = logN;
+    return;
+  }
+  const bool r16 = EPT >= 16 && logN % 3 != 0;
+  lfirst = r16 ? 4 : 3;
+  const int lmax = r16 ? 4 : 3;
+  int logNs = lfirst;
+  while (logN - logNs > lmax) logNs += 3;
+  llast = logN - logNs;
+}
+
 template <typename T, int DIR, int EPT, int kT, class Src, class Dst,
           class Fetch = NoFetch>
 __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
                                             int logN, int logL, Src&& src,
                                             Dst&& dst, const Fetch& fetch = Fetch()) {
   const int lmask = (1 << logL) - 1;
-  if (logN <= 3) {
+  int lfirst, llast;
+  tile_plan<EPT>(logN, lfirst, llast);
+  if (lfirst == 0) {
     // single stage: HBM -> registers -> HBM
     auto single = [&](auto Rc) {
       constexpr int R = decltype(Rc)::value;
@@ -78,19 +102,24 @@ __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
         }
       }
     };
+    if constexpr (EPT >= 16) {
+      if (logN == 4) single(std::integral_constant<int, 16>());
+    }
     if (logN == 3)
       single(std::integral_constant<int, 8>());
     else if (logN == 2)
       single(std::integral_constant<int, 4>());
-    else
+    else if (logN < 2)
       single(std::integral_constant<int, 2>());
     __syncthreads();
     return;
   }
-  // first stage, radix 8, Ns = 1: inputs from HBM, outputs to LDS
-  {
-    constexpr int NB = EPT / 8;
-    const int lognb = logN - 3;
+  // first stage, Ns = 1: inputs from HBM, outputs to LDS
+  auto first = [&](auto Rc) {
+    constexpr int R = decltype(Rc)::value;
+    constexpr int LOGR = R == 16 ? 4 : 3;
+    constexpr int NB = EPT / R;
+    const int lognb = logN - LOGR;
     const int total = 1 << (lognb + logL);
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -98,27 +127,31 @@ __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
       if (b < total) {
         const int line = b & lmask;
         const int j = b >> logL;
-        cpx<T> v[8];
+        cpx<T> v[R];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = src(i * 8 + r, line, j + (r << lognb));
-        dft8<T, DIR>(v);
+        for (int r = 0; r < R; ++r) v[r] = src(i * R + r, line, j + (r << lognb));
+        dftR<T, DIR, R>(v);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) buf[lpad((((j << 3) + r) << logL) + line)] = v[r];
+        for (int r = 0; r < R; ++r) buf[lpad((((j << LOGR) + r) << logL) + line)] = v[r];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) fetch(i * 8 + r, line, j + (r << lognb));
+        for (int r = 0; r < R; ++r) fetch(i * R + r, line, j + (r << lognb));
       }
     }
     __syncthreads();
+  };
+  if constexpr (EPT >= 16) {
+    if (lfirst == 4) first(std::integral_constant<int, 16>());
   }
-  int logNs = 3;
-  while (logN - logNs > 3) {
+  if (lfirst == 3) first(std::integral_constant<int, 8>());
+  int logNs = lfirst;
+  while (logN - logNs > llast) {
     lds_stage_lf<T, DIR, EPT, 8, kT>(buf, logN, logNs, logL, tw);
     logNs += 3;
   }
   // last stage: k = j, outputs at j + r Ns, to HBM
   auto last = [&](auto Rc) {
     constexpr int R = decltype(Rc)::value;
-    constexpr int LOGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+    constexpr int LOGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
     constexpr int NB = EPT / R;
     const int lognb = logN - LOGR;
     const int total = 1 << (lognb + logL);
@@ -138,12 +171,14 @@ __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
       }
     }
   };
-  const int rem = logN - logNs;
-  if (rem == 3)
+  if constexpr (EPT >= 16) {
+    if (llast == 4) last(std::integral_constant<int, 16>());
+  }
+  if (llast == 3)
     last(std::integral_constant<int, 8>());
-  else if (rem == 2)
+  else if (llast == 2)
     last(std::integral_constant<int, 4>());
-  else
+  else if (llast == 1)
     last(std::integral_constant<int, 2>());
   __syncthreads();
 }
@@ -199,14 +234,8 @@ __device__ __forceinline__ bool last_slot(int e, const StridedDesc<T>& d, int lo
     return true;
   }
   const int logN = d.logN;
-  int logR;
-  if (logN <= 3) {
-    logR = logN;
-  } else {
-    int logNs = 3;
-    while (logN - logNs > 3) logNs += 3;
-    logR = logN - logNs;
-  }
+  int lfirst, logR;
+  tile_plan<EPT>(logN, lfirst, logR);
   const int i = e >> logR, r = e & ((1 << logR) - 1);
   if (i >= (EPT >> logR)) return false;
   const int b = threadIdx.x + i * kT;
