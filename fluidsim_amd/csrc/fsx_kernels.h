@@ -44,14 +44,6 @@ struct NoFetch {
   __device__ __forceinline__ void operator()(int, int, int) const {}
 };
 
-// Full transform of one tile.  src(e, l, p) gives the input at position p
-// of local line l (e = register slot of the first stage, stable across
-// calls); dst(e, l, p, v) receives the output at position p (e = register
-// slot of the last stage).  fetch(e, l, p) is called for every first-stage
-// slot once that slot's input has been consumed: a multi-transform pass
-// issues there the global loads of its next transform (same slots), so
-// they are in flight during this transform's LDS stages.  Ends with a
-// barrier so buf can be reused.
 // Stage plan of tile_fft_p2 for a length 2^logN with EPT elements per
 // thread: log2 of the first stage's radix (0: a single stage of radix
 // 2^logN) and of the last stage's; the stages between are radix 8.  With
@@ -74,6 +66,14 @@ __device__ __forceinline__ void tile_plan(int logN, int& lfirst, int& llast) {
   llast = logN - logNs;
 }
 
+// Full transform of one tile.  src(e, l, p) gives the input at position p
+// of local line l (e = register slot of the first stage, stable across
+// calls); dst(e, l, p, v) receives the output at position p (e = register
+// slot of the last stage).  fetch(e, l, p) is called for every first-stage
+// slot once that slot's input has been consumed: a multi-transform pass
+// issues there the global loads of its next transform (same slots), so
+// they are in flight during this transform's LDS stages.  Ends with a
+// barrier so buf can be reused.
 template <typename T, int DIR, int EPT, int kT, class Src, class Dst,
           class Fetch = NoFetch>
 __device__ __forceinline__ void tile_fft_p2(cpx<T>* buf, const cpx<T>* tw,
