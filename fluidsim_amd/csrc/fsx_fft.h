@@ -66,6 +66,15 @@ __device__ __forceinline__ cpx<T> zero() {
   return mk<T>(T(0), T(0));
 }
 
+// a b + c with one rounding, whatever the fp contraction setting
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_fmaf(a, b, c);
+  else
+    return __builtin_fma(a, b, c);
+}
+
 // DIR = -1: forward transform, kernel exp(-2 pi i jk/N).
 // DIR = +1: inverse transform, kernel exp(+2 pi i jk/N).
 // Multiplication by exp(DIR * i pi / 2): forward -> -i, inverse -> +i.
@@ -138,10 +147,7 @@ __device__ __forceinline__ cpx<T> w16(cpx<T> a) {
     const T wi = DIR < 0 ? -sn : sn;
     // explicit fused multiply-adds: the rounding is then the same in every
     // pass that inlines this (no choice left to fp contraction)
-    if constexpr (sizeof(T) == 4)
-      return mk<T>(__builtin_fmaf(a.x, c, -(a.y * wi)), __builtin_fmaf(a.x, wi, a.y * c));
-    else
-      return mk<T>(__builtin_fma(a.x, c, -(a.y * wi)), __builtin_fma(a.x, wi, a.y * c));
+    return mk<T>(fma_t(a.x, c, -(a.y * wi)), fma_t(a.x, wi, a.y * c));
   }
 }
 

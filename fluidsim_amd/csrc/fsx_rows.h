@@ -407,9 +407,11 @@ void k_xrow(XRDesc<T> d, Op op) {
           Xk.y = T(0);
           Xm.y = T(0);
         }
+        // z = (Xk + conj Xm) + i conj(w) (Xk - conj Xm), as 4 adds + 4 FMAs
         const cpx<T> w = tw.pair[q];
-        const cpx<T> dk = Xk - conj(Xm);
-        z[q] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
+        const T ax = Xk.x + Xm.x, ay = Xk.y - Xm.y;
+        const T dx = Xk.x - Xm.x, dy = Xk.y + Xm.y;
+        z[q] = mk<T>(fma_t(w.y, dx, fma_t(-w.x, dy, ax)), fma_t(w.y, dy, fma_t(w.x, dx, ay)));
       }
       if (next >= 0) fetch(next);
       row_fft<T, +1, LOGM, LOGP>(z, xb, l, tw);
@@ -428,17 +430,27 @@ void k_xrow(XRDesc<T> d, Op op) {
       wave_sync();
       const __amdgpu_buffer_rsrc_t rd =
           buf_rsrc(d.out + fo * d.ofs + (long long)wrow0 * d.ld_out, out_bytes);
-      const T h = T(0.5);
+      // X[k] = s (E + P), X[M] = s conj(E - P) with E = (Zk + Zm) / 2,
+      // P = w (Zk - Zm) / (2 i), Zm = conj Z[M - k]: with a = Zk + Zm,
+      // dd = Zk - Zm and p = w (dd.y, -dd.x), X[k] = (s / 2) (a + p)
+      const T hs = T(0.5) * d.scale;
 #pragma unroll
       for (int i = 0; i < P; ++i) {
         const int q = PL::qpos(i);
         const int k = l + G * q;
         const cpx<T> Zk = v[i];
         const cpx<T> Zm = conj(xb[xpad<T>((M - k) & (M - 1))]);
-        const cpx<T> E = scale(Zk + Zm, h), dd = Zk - Zm;
-        const cpx<T> Pk = tw.pair[q] * mk<T>(dd.y * h, -dd.x * h);
-        buf_store<T>(rd, offo[q], scale(E + Pk, d.scale));
-        if (q == 0) buf_store<T>(rd, offn, scale(conj(E - Pk), d.scale));
+        const cpx<T> w = tw.pair[q];
+        const T ax = Zk.x + Zm.x, ay = Zk.y + Zm.y;
+        const T dx = Zk.x - Zm.x, dy = Zk.y - Zm.y;
+        if (q == 0) {
+          const T px = fma_t(w.x, dy, w.y * dx), py = fma_t(w.y, dy, -(w.x * dx));
+          buf_store<T>(rd, offo[q], mk<T>((ax + px) * hs, (ay + py) * hs));
+          buf_store<T>(rd, offn, mk<T>((ax - px) * hs, (py - ay) * hs));
+        } else {
+          buf_store<T>(rd, offo[q], mk<T>(fma_t(w.x, dy, fma_t(w.y, dx, ax)) * hs,
+                                          fma_t(w.y, dy, fma_t(-w.x, dx, ay)) * hs));
+        }
       }
     };
     if constexpr (Op::kTwoPhase) {
