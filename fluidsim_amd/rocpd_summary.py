@@ -70,7 +70,9 @@ def traffic(fetch_db, write_db, copy_mib, prefix=""):
     COPY_MIB MiB: factor = true KiB / counted KiB. The calibration copy is
     the first copyBuffer dispatch that counts at least 1 MiB (bench.py
     times its 1 GiB device copy before anything else runs; later copies,
-    e.g. of whole state fields, have other sizes)."""
+    e.g. of whole state fields, have other sizes).  A kernel's value is the
+    median over its dispatches (a dispatch whose counters come back short
+    does not drag it down); the range over the dispatches is printed."""
     import json
 
     true_kib = copy_mib * 1024.0
@@ -90,18 +92,25 @@ def traffic(fetch_db, write_db, copy_mib, prefix=""):
         for name, dur, val, grid in rows:
             agg[short_name(name)].append(val)
         for name, vals in agg.items():
-            out.setdefault(name, {})[counter] = sum(vals) / len(vals)
+            v = sorted(vals)
+            med = v[len(v) // 2] if len(v) % 2 else 0.5 * (v[len(v) // 2 - 1] + v[len(v) // 2])
+            out.setdefault(name, {})[counter] = med
+            out[name][counter + "_range"] = (v[0], v[-1], len(v))
     print(f"calibration: {copy_mib} MiB device copy; FETCH_SIZE factor "
           f"{factors['FETCH_SIZE']:.3f}, WRITE_SIZE factor {factors['WRITE_SIZE']:.3f}")
-    print(f"{'kernel':60s} {'FETCH KiB':>14s} {'WRITE KiB':>14s} {'HBM MB/launch':>14s}")
+    print(f"{'kernel':72s} {'FETCH KiB':>14s} {'WRITE KiB':>14s} {'HBM MB/launch':>14s}"
+          "  dispatches, FETCH and WRITE min-max KiB")
     res = {}
-    for name, c in sorted(out.items(), key=lambda kv: -sum(kv[1].values())):
+    key = lambda kv: -(kv[1].get("FETCH_SIZE", 0) + kv[1].get("WRITE_SIZE", 0))
+    for name, c in sorted(out.items(), key=key):
         if "FETCH_SIZE" not in c or "WRITE_SIZE" not in c:
             continue
         b = 1024.0 * (c["FETCH_SIZE"] * factors["FETCH_SIZE"]
                       + c["WRITE_SIZE"] * factors["WRITE_SIZE"])
         res[name] = b
-        print(f"{name[:60]:60s} {c['FETCH_SIZE']:14.1f} {c['WRITE_SIZE']:14.1f} {b / 1e6:14.1f}")
+        fr, wr = c["FETCH_SIZE_range"], c["WRITE_SIZE_range"]
+        print(f"{name[:72]:72s} {c['FETCH_SIZE']:14.1f} {c['WRITE_SIZE']:14.1f} {b / 1e6:14.1f}"
+              f"  {fr[2]}, {fr[0]:.0f}-{fr[1]:.0f}, {wr[0]:.0f}-{wr[1]:.0f}")
     if prefix:
         print(json.dumps({prefix + k: v for k, v in res.items()}))
     return res
