@@ -4,11 +4,14 @@
 
 namespace {
 
-// LDS of one block record is kept within 64 KiB (two blocks per CU)
-constexpr int kDiagLdsDoubles = 8192;
+// LDS of one block record is kept within 128 KiB (one 512-thread block per
+// CU): the five quantities of a 1024^3 spectrum fit one record, so the
+// input is read once
+constexpr int kDiagLdsDoubles = 16384;
 
 // histogram copies per block (lane parity spreads same-bin adds) and the
-// LDS budget of a block record with its copies (two blocks per CU)
+// LDS budget of a block record with its copies (two blocks per CU; larger
+// records get no copies)
 constexpr int kDiagCopies = 4;
 constexpr size_t kDiagLdsBytes = 80 * 1024;
 
@@ -116,17 +119,23 @@ int diag_launch(const DiagIn& in, const Box& box, const DiagGeom& g, double* par
   note(K_DIAG, (double)nin * g.nz * g.ny * g.nkx * sizeof(cpx<T>) +
                    (double)nb * g.rec * sizeof(double));
   const size_t lds = diag_lds(g);
-  static size_t lds_set = 0;
-  if (lds > lds_set) {
-    int rc = set_lds(k_diag<T, KIND>, lds);
+  // records over 64 KiB leave room for one block per CU: 512 threads then,
+  // 256 otherwise (two or more blocks per CU)
+  auto go = [&](auto kern, unsigned nt, size_t& lds_set) {
+    if (lds > lds_set) {
+      int rc = set_lds(kern, lds);
+      if (rc) return rc;
+      lds_set = lds;
+    }
+    int rc = tm_pre(s);
     if (rc) return rc;
-    lds_set = lds;
-  }
-  int rc = tm_pre(s);
-  if (rc) return rc;
-  hipLaunchKernelGGL((k_diag<T, KIND>), dim3(nb), dim3(256), lds, s, in, box, g, part);
-  FSX_HIP(hipGetLastError());
-  return tm_post(s);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(nt), lds, s, in, box, g, part);
+    FSX_HIP(hipGetLastError());
+    return tm_post(s);
+  };
+  static size_t lds_small = 0, lds_big = 0;
+  if (lds > 64 * 1024) return go(k_diag<T, KIND, 512>, 512, lds_big);
+  return go(k_diag<T, KIND, 256>, 256, lds_small);
 }
 
 template <typename T>

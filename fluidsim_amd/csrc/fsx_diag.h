@@ -332,8 +332,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <typename T, int KIND>
-static __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeom g,
+// NT threads per block (waves take the block's ky rows in turn): 256, or
+// 512 for records that leave room for one block per CU
+template <typename T, int KIND, int NT>
+static __global__ __launch_bounds__(NT) void k_diag(DiagIn in, Box box, DiagGeom g,
                                               double* partial) {
 #pragma clang fp contract(off)
   constexpr int Q = diag_nq(KIND);
@@ -358,20 +360,79 @@ static __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeo
   double tot[Q];
 #pragma unroll
   for (int s = 0; s < Q; ++s) tot[s] = 0.0;
+  // With shell or kh histograms, lanes own runs of kDiagRun consecutive kx
+  // (blocks of 64 kDiagRun modes): along a run |k| and kh grow slowly, so
+  // each lane sums the shares of its modes in registers while the bin stays
+  // the same and adds them to the LDS histograms when it moves on (and at
+  // the end of the row), instead of two LDS atomics per mode that
+  // neighbouring lanes aim at the same bin.  A lane's loads are then
+  // kDiagRun elements apart (the block is still read in full).  Without
+  // those histograms lanes take consecutive kx.
+  // (kinds of at most 6 quantities: the means kinds carry 9-18 and bin
+  // nothing, the strat budget's 8 running sums would spill; those keep one
+  // lane per kx and direct atomics)
+  constexpr int kDiagRun = 4;
+  constexpr bool kRuns = Q <= 6;
+  const bool runs = kRuns && (g.flags & (DIAG_SHELL | DIAG_KZKH)) != 0;
   for (int iy = y0 + wid; iy < y1; iy += nw) {
     m.iky = ksigned(iy, g.ny);
     m.ky = m.iky * g.dky;
     double row[Q];
 #pragma unroll
     for (int s = 0; s < Q; ++s) row[s] = 0.0;
-    for (int ikx = lane; ikx < g.nkx; ikx += 64) {
-      m.ikx = ikx;
-      m.kx = ikx * g.dkx;
-      m.k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+    // running bins (-1: none yet) and the sums for bins j and j + 1
+    constexpr int QR = kRuns ? Q : 1;
+    int js = -1, jh = -1;
+    double s0[QR], s1[QR], h0[QR], h1[QR];
+#pragma unroll
+    for (int s = 0; s < QR; ++s) s0[s] = s1[s] = h0[s] = h1[s] = 0.0;
+    // adds (a0, a1) to bins (j, j + 1) of histogram off (n bins)
+    auto flush = [&](int off, int n, int j, const double(&a0)[QR], const double(&a1)[QR]) {
+#pragma unroll
+      for (int s = 0; s < Q; ++s)
+        if (s >= g.q0 && s < g.q0 + g.nq) {
+          double* h = &lds[hshift + off + (s - g.q0) * n];
+          atomicAdd(h + j, a0[s]);
+          if (j + 1 < n && a1[s] != 0.0) atomicAdd(h + j + 1, a1[s]);
+        }
+    };
+    // mode with bin j and share c of value q into the running sums
+    auto accumulate = [&](int off, int n, int j, double c, const double(&q)[Q], int& jc,
+                          double(&a0)[QR], double(&a1)[QR]) {
+      if (j != jc) {
+        if (jc >= 0 && j == jc + 1) {
+          // bin jc is complete; its successor's sum so far moves down
+#pragma unroll
+          for (int s = 0; s < Q; ++s)
+            if (s >= g.q0 && s < g.q0 + g.nq)
+              atomicAdd(&lds[hshift + off + (s - g.q0) * n + jc], a0[s]);
+#pragma unroll
+          for (int s = 0; s < Q; ++s) {
+            a0[s] = a1[s];
+            a1[s] = 0.0;
+          }
+        } else {
+          if (jc >= 0) flush(off, n, jc, a0, a1);
+#pragma unroll
+          for (int s = 0; s < Q; ++s) a0[s] = a1[s] = 0.0;
+        }
+        jc = j;
+      }
+#pragma unroll
+      for (int s = 0; s < Q; ++s) {
+        a0[s] += (1.0 - c) * q[s];
+        a1[s] += c * q[s];
+      }
+    };
+    auto mode = [&](int ikx) {
+      Mode mm = m;  // (a copy: the row's m stays out of the lambda's reach)
+      mm.ikx = ikx;
+      mm.kx = ikx * g.dkx;
+      mm.k2 = mm.kx * mm.kx + mm.ky * mm.ky + mm.kz * mm.kz;
       const size_t i = ((size_t)iz * g.ny + iy) * g.nkx + ikx;
       const double w = (ikx == 0 || (even && ikx == g.nx / 2)) ? 1.0 : 2.0;
       double q[Q];
-      diag_eval<KIND, T>(in, box, i, m, q);
+      diag_eval<KIND, T>(in, box, i, mm, q);
 #pragma unroll
       for (int s = 0; s < Q; ++s) {
         q[s] *= w;
@@ -386,27 +447,76 @@ static __global__ __launch_bounds__(256) void k_diag(DiagIn in, Box box, DiagGeo
       if (g.flags & DIAG_SHELL) {
         int j;
         double c;
-        share_bin(sqrt(m.k2), g.dk3, g.nk3, j, c);
-#pragma unroll
-        for (int s = 0; s < Q; ++s)
-          if (s >= g.q0 && s < g.q0 + g.nq) {
-            double* h = &lds[hshift + g.off_k3 + (s - g.q0) * g.nk3];
-            atomicAdd(h + j, (1.0 - c) * q[s]);
-            if (c != 0.0) atomicAdd(h + j + 1, c * q[s]);
-          }
+        share_bin(sqrt(mm.k2), g.dk3, g.nk3, j, c);
+        accumulate(g.off_k3, g.nk3, j, c, q, js, s0, s1);
       }
       if (g.flags & DIAG_KZKH) {
         int j;
         double c;
-        share_bin(sqrt(m.kx * m.kx + m.ky * m.ky), g.dkh, g.nkh, j, c);
-#pragma unroll
-        for (int s = 0; s < Q; ++s)
-          if (s >= g.q0 && s < g.q0 + g.nq) {
-            double* h = &lds[hshift + g.off_kh + (s - g.q0) * g.nkh];
-            atomicAdd(h + j, (1.0 - c) * q[s]);
-            if (c != 0.0) atomicAdd(h + j + 1, c * q[s]);
-          }
+        share_bin(sqrt(mm.kx * mm.kx + mm.ky * mm.ky), g.dkh, g.nkh, j, c);
+        accumulate(g.off_kh, g.nkh, j, c, q, jh, h0, h1);
       }
+    };
+    if constexpr (kRuns) {
+      if (runs) {
+        for (int blk0 = 0; blk0 < g.nkx; blk0 += kDiagRun * 64)
+#pragma unroll
+          for (int r = 0; r < kDiagRun; ++r) {
+            const int ikx = blk0 + lane * kDiagRun + r;
+            if (ikx < g.nkx) mode(ikx);
+          }
+      } else {
+        for (int ikx = lane; ikx < g.nkx; ikx += 64) mode(ikx);
+      }
+    } else {
+      for (int ikx = lane; ikx < g.nkx; ikx += 64) {
+        m.ikx = ikx;
+        m.kx = ikx * g.dkx;
+        m.k2 = m.kx * m.kx + m.ky * m.ky + m.kz * m.kz;
+        const size_t i = ((size_t)iz * g.ny + iy) * g.nkx + ikx;
+        const double w = (ikx == 0 || (even && ikx == g.nx / 2)) ? 1.0 : 2.0;
+        double q[Q];
+        diag_eval<KIND, T>(in, box, i, m, q);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+          q[s] *= w;
+          row[s] += q[s];
+        }
+        if (g.flags & DIAG_KX) {
+#pragma unroll
+          for (int s = 0; s < Q; ++s)
+            if (s >= g.q0 && s < g.q0 + g.nq)
+              atomicAdd(&lds[g.off_kx + (s - g.q0) * g.nkxs + ikx], q[s]);
+        }
+        if (g.flags & DIAG_SHELL) {
+          int j;
+          double c;
+          share_bin(sqrt(m.k2), g.dk3, g.nk3, j, c);
+#pragma unroll
+          for (int s = 0; s < Q; ++s)
+            if (s >= g.q0 && s < g.q0 + g.nq) {
+              double* h = &lds[hshift + g.off_k3 + (s - g.q0) * g.nk3];
+              atomicAdd(h + j, (1.0 - c) * q[s]);
+              if (c != 0.0) atomicAdd(h + j + 1, c * q[s]);
+            }
+        }
+        if (g.flags & DIAG_KZKH) {
+          int j;
+          double c;
+          share_bin(sqrt(m.kx * m.kx + m.ky * m.ky), g.dkh, g.nkh, j, c);
+#pragma unroll
+          for (int s = 0; s < Q; ++s)
+            if (s >= g.q0 && s < g.q0 + g.nq) {
+              double* h = &lds[hshift + g.off_kh + (s - g.q0) * g.nkh];
+              atomicAdd(h + j, (1.0 - c) * q[s]);
+              if (c != 0.0) atomicAdd(h + j + 1, c * q[s]);
+            }
+        }
+      }
+    }
+    if constexpr (kRuns) {
+      if (js >= 0) flush(g.off_k3, g.nk3, js, s0, s1);
+      if (jh >= 0) flush(g.off_kh, g.nkh, jh, h0, h1);
     }
     if (g.flags & DIAG_KY) {
       const int jy = m.iky < 0 ? -m.iky : m.iky;
