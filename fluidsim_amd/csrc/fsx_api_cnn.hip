@@ -49,7 +49,7 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   int rc;
 #define FSX_CONV_FWD(CI_, F_, XF_)                                                      \
   if (ci == CI_ && fuse == F_ && x_f32 == XF_) {                                          \
-    if ((rc = set_lds(k_conv_fwd<CI_, F_, XF_>, conv_fwd_lds(CI_, W, kCnnRows)))) return rc; \
+    if ((rc = set_lds(k_conv_fwd<CI_, F_, XF_>, lds))) return rc;                        \
     if ((rc = tm_pre(s))) return rc;                                                      \
     conv_fwd_launch<CI_, F_, XF_>(nblk, lds, s, xi, wi, bias, y, H, W, co_out, R, ai);    \
   }

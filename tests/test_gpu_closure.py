@@ -187,3 +187,33 @@ def test_closure_term_one_call():
     sim64.closure_net = net
     with pytest.raises(ValueError):
         sim64._closure_term(sim64.state.state_spect.data)
+
+
+@pytest.mark.parametrize("n,nlayers", [(128, 6), (256, 6), (64, 2)])
+def test_closure_term_les_sizes(n, nlayers):
+    """The closure term equals the composition of its pieces (c2r, network
+    forward, r2c, dealiasing) bit for bit at the LES sizes, including 256^2,
+    whose convolution tiles of one row need a smaller LDS request than the
+    4-row tile of larger batches, and with the smallest network; repeated
+    calls reuse the workspace."""
+    print(f"case closure term n={n} layers={nlayers}", flush=True)
+    SimulLES = CL.make_les_class()
+    _, params = CL.ns2d_params(n, 1e-3, forcing=False, les=True)
+    sim = SimulLES(params)
+    sim.state.init_statespect_from(rot_fft=ns2d_noise_rot_fft(n, n, seed=11, kmax=24))
+    net = CL.CNNClosure(n_in=1, nlayers=nlayers, seed=5)
+    rng = np.random.default_rng(12)
+    for t in net.b:
+        t.copy_(torch.from_numpy(rng.standard_normal(t.numel()).astype(np.float32) * 0.1))
+    sim.closure_net = net
+    S = sim.state.state_spect.data
+    for _ in range(3):
+        got = sim._closure_term(S).clone()
+        x = torch.empty((1, n, n, 1), dtype=torch.float32, device="cuda")
+        sim.oper.ifft_as_arg(S[0], x.view(n, n))
+        y = net.forward(x)
+        ref = torch.empty_like(got)
+        sim.oper.fft_as_arg(y.view(n, n), ref[0])
+        sim.oper.dealiasing(ref)
+        assert torch.equal(got, ref)
+        assert float(got.abs().max()) > 0
