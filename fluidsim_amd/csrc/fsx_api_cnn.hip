@@ -113,16 +113,25 @@ int cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
 // by the caller.
 int cnn_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W, float* dw,
               float* dbias, float* part, hipStream_t s) {
-  const size_t lds = conv_wgrad_lds(ci, W);
+  // row blocks of kWgRowsMax rows where their LDS fits and H divides
+  const bool wide = conv_wgrad_lds(ci, W, kWgRowsMax) <= kLdsMax && H % kWgRowsMax == 0;
+  const int rows = wide ? kWgRowsMax : kWgRows;
+  const size_t lds = conv_wgrad_lds(ci, W, rows);
   if (lds > kLdsMax) return fail(FSX_ERR_ARG, "image too wide for the weight-gradient tile");
-  const int groups = wgrad_groups(B, H);
+  const int groups = wgrad_groups(B, H, rows);
   const auto* xi = reinterpret_cast<const uint16_t*>(x);
   const auto* di = reinterpret_cast<const uint16_t*>(dy);
   int rc;
-#define FSX_WG(CI_, CO_)                                                              \
-  if ((rc = set_lds(k_conv_wgrad<CI_, CO_>, lds))) return rc;                          \
-  hipLaunchKernelGGL((k_conv_wgrad<CI_, CO_>), dim3(groups), dim3(kWgThreads), lds, s, xi, di, \
-                     part, B, H, W);
+#define FSX_WG(CI_, CO_)                                                                    \
+  if (wide) {                                                                               \
+    if ((rc = set_lds(k_conv_wgrad<CI_, CO_, kWgRowsMax>, lds))) return rc;                 \
+    hipLaunchKernelGGL((k_conv_wgrad<CI_, CO_, kWgRowsMax>), dim3(groups), dim3(kWgThreads), \
+                       lds, s, xi, di, part, B, H, W);                                      \
+  } else {                                                                                  \
+    if ((rc = set_lds(k_conv_wgrad<CI_, CO_, kWgRows>, lds))) return rc;                    \
+    hipLaunchKernelGGL((k_conv_wgrad<CI_, CO_, kWgRows>), dim3(groups), dim3(kWgThreads),   \
+                       lds, s, xi, di, part, B, H, W);                                      \
+  }
   if (ci == 8 && co_in == 8) {
     FSX_WG(8, 8)
   } else if (ci == 8) {

@@ -258,10 +258,9 @@ static __global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad
 // dimension) on v_mfma_f32_32x32x16_bf16: M = 32 output channels o,
 // N = 32 input channels c of one tap.
 //
-// Row blocks of kWgRows output rows.  Staging transposes to pixels-fast in
-// LDS: dy as [kWgRows][32 o][DP], the x patch (rows y0 - 2 .. y0 + kWgRows
-// + 1, patch column j = image column j - 2, periodic) as
-// [kWgRows + 4][32 c][XP].  Channel row c starts wg_pad(c) elements into
+// Row blocks of ROWS output rows.  Staging transposes to pixels-fast in
+// LDS: dy as [ROWS][32 o][DP], the x patch (rows y0 - 2 .. y0 + ROWS + 1,
+// patch column j = image column j - 2, periodic) as [ROWS + 4][32 c][XP].  Channel row c starts wg_pad(c) elements into
 // its slot and each patch row is 32 elements longer than its 32 channel
 // rows, so the transposed 8-byte writes of the lanes of one pixel run (4
 // channel vectors) spread over the banks; channel rows beyond CI / CO_IN
@@ -281,11 +280,16 @@ static __global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad
 //
 // Persistent: workgroup g (of G) takes a contiguous run of row blocks of
 // its XCD's contiguous share and keeps the x patch rows in a ring of
-// kWgRows + 4 slots, so consecutive row blocks of an image stage kWgRows new
-// x rows instead of kWgRows + 4; it writes its partials part[g][32][25 CI]
+// ROWS + 4 slots, so consecutive row blocks of an image stage ROWS new x
+// rows instead of ROWS + 4; it writes its partials part[g][32][25 CI]
 // and, after all G weight partials, part[G 32 25 CI + 32 g + o];
 // k_cnn_reduce sums them in a fixed order (deterministic).
-constexpr int kWgRows = 2;      // output rows per row block
+// Rows per block ROWS: kWgRowsMax where the block's LDS fits (fewer x rows
+// staged per output row and half the blocks: batch 32 of 128^2, 63 -> 55 us
+// with the partials' reduction), else kWgRows, which also bounds the
+// workgroup count of the workspace.
+constexpr int kWgRowsMax = 4;
+constexpr int kWgRows = 2;
 constexpr int kWgThreads = 320; // 5 waves, one per tap row
 constexpr int kWgGroups = 256;  // persistent workgroups (one per CU; halves the partials)
 
@@ -295,14 +299,15 @@ __host__ __device__ inline int wg_pad(int c) { return 8 * ((c >> 3) & 3); }
 // LDS elements of one dy row / one x patch row (32 channel rows + pads)
 __host__ __device__ inline int wg_drow(int W) { return 32 * wg_dp(W) + 32; }
 __host__ __device__ inline int wg_xrow(int W) { return 32 * wg_xp(W) + 32; }
-// LDS bytes of k_conv_wgrad
-__host__ __device__ inline size_t conv_wgrad_lds(int ci, int W) {
+// LDS bytes of k_conv_wgrad with row blocks of `rows` rows
+__host__ __device__ inline size_t conv_wgrad_lds(int ci, int W, int rows = kWgRows) {
   (void)ci;
-  return (size_t)2 * ((size_t)kWgRows * wg_drow(W) + (size_t)(kWgRows + 4) * wg_xrow(W));
+  return (size_t)2 * ((size_t)rows * wg_drow(W) + (size_t)(rows + 4) * wg_xrow(W));
 }
-// Workgroups of k_conv_wgrad for B images of H rows.
-inline int wgrad_groups(long long B, int H) {
-  const long long nblk = B * (H / kWgRows);
+// Workgroups of k_conv_wgrad for B images of H rows (at most: row blocks of
+// kWgRows rows; blocks of more rows give at most as many)
+inline int wgrad_groups(long long B, int H, int rows = kWgRows) {
+  const long long nblk = B * (H / rows);
   if (nblk >= kWgGroups) return kWgGroups;
   return (int)nblk;
 }
@@ -348,14 +353,14 @@ __device__ __forceinline__ void wg_chunk(const u32x4& a, const u32x4& x0, const 
 // MFMA loop over the staged row block: full 16-pixel chunks with the loads
 // of the next chunk issued before the MFMAs of the current one, then, for
 // W % 16 == 8, the last 8 pixels (upper half-wave supplies zeros).
-template <bool BIAS>
-__device__ __forceinline__ void wg_rows(const uint16_t* pa0, const uint16_t* const (&pxrow)[kWgRows],
+template <bool BIAS, int ROWS>
+__device__ __forceinline__ void wg_rows(const uint16_t* pa0, const uint16_t* const (&pxrow)[ROWS],
                                         int W, int DROW, int g, f32x16 (&acc)[5],
                                         f32x16& accb) {
   const u32x4 ones = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};
   const int wfull = W & ~15;
 #pragma unroll
-  for (int r = 0; r < kWgRows; ++r) {
+  for (int r = 0; r < ROWS; ++r) {
     const uint16_t* pa = pa0 + r * DROW;
     const uint16_t* pxr = pxrow[r];
     u32x4 a = *reinterpret_cast<const u32x4*>(pa);
@@ -384,7 +389,7 @@ __device__ __forceinline__ void wg_rows(const uint16_t* pa0, const uint16_t* con
   }
 }
 
-template <int CI, int CO_IN>
+template <int CI, int CO_IN, int ROWS>
 static __global__ __launch_bounds__(kWgThreads) __attribute__((amdgpu_waves_per_eu(3)))
 void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                   float* __restrict__ part, int B, int H, int W) {
@@ -393,15 +398,15 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
   extern __shared__ __align__(16) uint16_t cnn_sm[];
   const int DP = wg_dp(W), XP = wg_xp(W);
   const int DROW = wg_drow(W), XROW = wg_xrow(W);
-  uint16_t* sd = cnn_sm;                    // [kWgRows][DROW]
-  uint16_t* sx = cnn_sm + kWgRows * DROW;   // [kWgRows + 4][XROW]
+  uint16_t* sd = cnn_sm;                    // [ROWS][DROW]
+  uint16_t* sx = cnn_sm + ROWS * DROW;   // [ROWS + 4][XROW]
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int m = lane & 31, g = lane >> 5;
   // channel rows that staging never writes: zero once
   if (CI < 32 || CO_IN < 32) {
     const u32x4 z = {0u, 0u, 0u, 0u};
-    for (int r = 0; r < kWgRows + 4; ++r) {
-      if (r < kWgRows)
+    for (int r = 0; r < ROWS + 4; ++r) {
+      if (r < ROWS)
         for (int i = threadIdx.x; i < (32 - CO_IN) * (DP / 8); i += kWgThreads) {
           const int o = CO_IN + i / (DP / 8);
           *reinterpret_cast<u32x4*>(sd + r * DROW + o * DP + wg_pad(o) + 8 * (i % (DP / 8))) = z;
@@ -415,7 +420,7 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
   f32x16 acc[5], accb = {};
 #pragma unroll
   for (int q = 0; q < 5; ++q) acc[q] = f32x16{};
-  const int nyb = H / kWgRows;
+  const int nyb = H / ROWS;
   const int nblk = B * nyb;
   const int G = gridDim.x;
   // a contiguous run of row blocks per workgroup, the runs of one XCD's
@@ -436,14 +441,14 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
   // this lane's dy operand row (pixels-fast): channel m
   const uint16_t* pa0 = sd + m * DP + wg_pad(m) + 8 * g;
   const int xoff = m * XP + wg_pad(m) + 8 * g;
-  // x rows live in a ring of kWgRows + 4 slots: image row y (unwrapped,
+  // x rows live in a ring of ROWS + 4 slots: image row y (unwrapped,
   // y >= -2) in slot (y + 6) mod 6; the next row block of the same image
   // keeps 4 rows and stages 2
-  constexpr int NSLOT = kWgRows + 4;
+  constexpr int NSLOT = ROWS + 4;
   int have_b = -1, have_y0 = 0;
   for (int blk = first; blk < last; ++blk) {
-    const int b = blk / nyb, y0 = (blk - b * nyb) * kWgRows;
-    const int pr0 = (b == have_b && y0 == have_y0 + kWgRows) ? NSLOT - kWgRows : 0;
+    const int b = blk / nyb, y0 = (blk - b * nyb) * ROWS;
+    const int pr0 = (b == have_b && y0 == have_y0 + ROWS) ? NSLOT - ROWS : 0;
     have_b = b;
     have_y0 = y0;
     __syncthreads();
@@ -472,7 +477,7 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
       }
     }
     // dy rows: item = (row, column group, channel vector)
-    for (int it = threadIdx.x; it < kWgRows * NVO * (W / 8); it += kWgThreads) {
+    for (int it = threadIdx.x; it < ROWS * NVO * (W / 8); it += kWgThreads) {
       const int q = it % NVO, rk = it / NVO;
       const int k = rk % (W / 8), r = rk / (W / 8);
       const uint16_t* src = dy + (((size_t)b * H + y0 + r) * W + 8 * k) * CO_IN + 8 * q;
@@ -491,14 +496,14 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
     }
     __syncthreads();
     // output row r, tap row wid: image row y0 + r + wid - 2
-    const uint16_t* pxrow[kWgRows];
+    const uint16_t* pxrow[ROWS];
 #pragma unroll
-    for (int r = 0; r < kWgRows; ++r)
+    for (int r = 0; r < ROWS; ++r)
       pxrow[r] = sx + ((y0 - 2 + r + wid + NSLOT) % NSLOT) * XROW + xoff;
     if (wid == 0)
-      wg_rows<true>(pa0, pxrow, W, DROW, g, acc, accb);
+      wg_rows<true, ROWS>(pa0, pxrow, W, DROW, g, acc, accb);
     else
-      wg_rows<false>(pa0, pxrow, W, DROW, g, acc, accb);
+      wg_rows<false, ROWS>(pa0, pxrow, W, DROW, g, acc, accb);
   }
   // partials: n = (5 w + dx) CI + c, lanes m < CI
   float* dst = part + (size_t)blockIdx.x * 32 * N;

@@ -107,7 +107,9 @@ def test_input_gradient_by_flip(ci, co, co_pad):
                                             # more row blocks than workgroups: each
                                             # workgroup keeps its x rows in the ring
                                             (32, 32, (16, 128, 64)), (8, 32, (20, 64, 32)),
-                                            (32, 8, (9, 128, 40))])
+                                            (32, 8, (9, 128, 40)),
+                                            # too wide for 4-row blocks: 2-row blocks
+                                            (32, 32, (1, 8, 256)), (8, 32, (2, 12, 248))])
 def test_weight_gradient(ci, co_in, shape):
     rng = np.random.default_rng(3)
     x = rounded(rng.standard_normal(shape + (ci,)))
