@@ -19,8 +19,14 @@ template <int CI, int FUSE, bool XF32>
 void conv_fwd_launch(unsigned nblk, size_t lds, hipStream_t s, const uint16_t* x,
                      const uint16_t* w, const float* bias, float* y, int H, int W, int co_out,
                      int R, uint16_t* a) {
-  hipLaunchKernelGGL((k_conv_fwd<CI, FUSE, XF32>), dim3(nblk), dim3(256), lds, s, x, w, bias, y,
-                     H, W, co_out, R, a);
+  // blocks of several rows: 8 waves (more tiles in flight per block; the
+  // LDS allows one block per CU), one-row blocks: 4 waves
+  if (R >= 2)
+    hipLaunchKernelGGL((k_conv_fwd<CI, FUSE, XF32, 512>), dim3(nblk), dim3(512), lds, s, x, w,
+                       bias, y, H, W, co_out, R, a);
+  else
+    hipLaunchKernelGGL((k_conv_fwd<CI, FUSE, XF32, 256>), dim3(nblk), dim3(256), lds, s, x, w,
+                       bias, y, H, W, co_out, R, a);
 }
 
 // x_f32: x is one FP32 channel (ci = 8 layer: padded while staged)
@@ -49,7 +55,9 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   int rc;
 #define FSX_CONV_FWD(CI_, F_, XF_)                                                      \
   if (ci == CI_ && fuse == F_ && x_f32 == XF_) {                                          \
-    if ((rc = set_lds(k_conv_fwd<CI_, F_, XF_>, lds))) return rc;                        \
+    rc = R >= 2 ? set_lds(k_conv_fwd<CI_, F_, XF_, 512>, lds)                             \
+                : set_lds(k_conv_fwd<CI_, F_, XF_, 256>, lds);                            \
+    if (rc) return rc;                                                                    \
     if ((rc = tm_pre(s))) return rc;                                                      \
     conv_fwd_launch<CI_, F_, XF_>(nblk, lds, s, xi, wi, bias, y, H, W, co_out, R, ai);    \
   }

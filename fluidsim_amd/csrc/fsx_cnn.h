@@ -118,8 +118,10 @@ __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
 // instead (co_out = 32); FUSE = 2 writes both.
 // XF32 (CI = 8 only): x is one FP32 channel x[B][H][W], rounded to BF16 and
 // zero-padded to 8 channels while it is staged.
-template <int CI, int FUSE = 0, bool XF32 = false>
-static __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ x,
+// NT threads: 256 (4 waves) for one-row blocks, 512 (8 waves, two tiles
+// each per row of 128 columns) for blocks of 2-4 rows
+template <int CI, int FUSE = 0, bool XF32 = false, int NT = 256>
+static __global__ __launch_bounds__(NT) void k_conv_fwd(const uint16_t* __restrict__ x,
                                                   const uint16_t* __restrict__ w,
                                                   const float* __restrict__ bias,
                                                   float* __restrict__ y, int H, int W,
