@@ -19,9 +19,24 @@ template <int CI, int FUSE, bool XF32>
 void conv_fwd_launch(unsigned nblk, size_t lds, hipStream_t s, const uint16_t* x,
                      const uint16_t* w, const float* bias, float* y, int H, int W, int co_out,
                      int R, uint16_t* a) {
-  // blocks of several rows: 8 waves (more tiles in flight per block; the
-  // LDS allows one block per CU), one-row blocks: 4 waves
-  if (R >= 2)
+  // 4-row blocks, more than two per CU: the persistent form (weights staged
+  // once per workgroup, patch rows in a ring), one workgroup per CU;
+  // blocks of 2 rows: 8 waves (more tiles in flight per block; the LDS
+  // allows one block per CU), one-row blocks: 4 waves
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+      ncu = 256;
+  }
+  if (R == kCnnRows && nblk > 2u * (unsigned)ncu) {
+    const int B = (int)(nblk / (unsigned)(H / R));
+    const unsigned grid = (unsigned)(ncu >= 8 ? ncu / 8 * 8 : ncu);
+    hipLaunchKernelGGL((k_conv_fwd_run<CI, FUSE, XF32>), dim3(grid), dim3(512), lds, s, x, w,
+                       bias, y, B, H, W, co_out, a);
+  } else if (R >= 2)
     hipLaunchKernelGGL((k_conv_fwd<CI, FUSE, XF32, 512>), dim3(nblk), dim3(512), lds, s, x, w,
                        bias, y, H, W, co_out, R, a);
   else
@@ -57,6 +72,7 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   if (ci == CI_ && fuse == F_ && x_f32 == XF_) {                                          \
     rc = R >= 2 ? set_lds(k_conv_fwd<CI_, F_, XF_, 512>, lds)                             \
                 : set_lds(k_conv_fwd<CI_, F_, XF_, 256>, lds);                            \
+    if (!rc && R == kCnnRows) rc = set_lds(k_conv_fwd_run<CI_, F_, XF_>, lds);            \
     if (rc) return rc;                                                                    \
     if ((rc = tm_pre(s))) return rc;                                                      \
     conv_fwd_launch<CI_, F_, XF_>(nblk, lds, s, xi, wi, bias, y, H, W, co_out, R, ai);    \

@@ -243,6 +243,169 @@ static __global__ __launch_bounds__(NT) void k_conv_fwd(const uint16_t* __restri
   }
 }
 
+// Persistent form of k_conv_fwd for batches (row blocks of kCnnRows = 4
+// rows, 8 waves): workgroup g takes a contiguous run of row blocks
+// (image-major) of its XCD's contiguous share, stages the 32 x K weights
+// once, and keeps the patch rows in a ring of 8 slots (unwrapped image row
+// yu >= -2 in slot (yu + 8) mod 8), so the next row block of the same
+// image stages its 4 new rows only.  Same arithmetic per output as
+// k_conv_fwd.
+template <int CI, int FUSE, bool XF32>
+static __global__ __launch_bounds__(512) void k_conv_fwd_run(
+    const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+    const float* __restrict__ bias, float* __restrict__ y, int B, int H, int W, int co_out,
+    uint16_t* __restrict__ a_out) {
+  constexpr int R = kCnnRows;
+  constexpr int NSLOT = R + 4;
+  static_assert(NSLOT == 8, "ring of 8 patch rows");
+  constexpr int K = kCnnTaps * CI;
+  constexpr int KP = cnn_kp(CI);
+  constexpr int PS = cnn_ps(CI);
+  constexpr int NV = CI / 8;  // 16-byte vectors per pixel
+  constexpr int kLoads = 16;
+  extern __shared__ __align__(16) uint16_t cnn_sm[];
+  uint16_t* sw = cnn_sm;
+  uint16_t* sp = cnn_sm + 32 * KP;
+  const int WP = W + 4;
+  const int nt = blockDim.x;
+  const int nyb = H / R;
+  const int nblk = B * nyb;
+  const int G = gridDim.x;
+  int first, last;
+  if (G % 8 == 0) {
+    const int gx = G / 8, xcd = blockIdx.x % 8, i = blockIdx.x / 8;
+    const int share = (nblk + 7) / 8, per = (share + gx - 1) / gx;
+    first = xcd * share + i * per;
+    last = min(min(first + per, (xcd + 1) * share), nblk);
+  } else {
+    const int per = (nblk + G - 1) / G;
+    first = blockIdx.x * per;
+    last = min(first + per, nblk);
+  }
+  // weights once: 32 rows of K, 16-byte vectors
+  {
+    constexpr int NW16 = 32 * K / 8;
+    for (int t0 = threadIdx.x; t0 < NW16; t0 += kLoads * nt) {
+      u32x4 vw[kLoads];
+#pragma unroll
+      for (int j = 0; j < kLoads; ++j) {
+        const int t = t0 + j * nt;
+        if (t < NW16) vw[j] = reinterpret_cast<const u32x4*>(w)[t];
+      }
+#pragma unroll
+      for (int j = 0; j < kLoads; ++j) {
+        const int t = t0 + j * nt;
+        if (t < NW16) {
+          const int co = (8 * t) / K, k = (8 * t) % K;
+          *reinterpret_cast<u32x4*>(&sw[co * KP + k]) = vw[j];
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int m = lane & 31, g = lane >> 5;
+  const int nchunk = W / 32;
+  int have_b = -1, have_y0 = 0;
+  for (int blk = first; blk < last; ++blk) {
+    const int b = blk / nyb, y0 = (blk - b * nyb) * R;
+    // patch rows pr0 .. NSLOT - 1 (rows y0 - 2 + pr) are staged
+    const int pr0 = (b == have_b && y0 == have_y0 + R) ? NSLOT - R : 0;
+    have_b = b;
+    have_y0 = y0;
+    __syncthreads();  // the previous block's MFMA reads of the ring are done
+    {
+      // element t = ((pr - pr0) WP + c) NV + q as in k_conv_fwd, the row
+      // going to its ring slot
+      const int NP16 = (NSLOT - pr0) * WP * NV;
+      const int q = threadIdx.x % NV, cstep = nt / NV;
+      int pc = (threadIdx.x / NV) % WP, pr = pr0 + (threadIdx.x / NV) / WP;
+      for (int t0 = threadIdx.x; t0 < NP16; t0 += kLoads * nt) {
+        u32x4 vp[kLoads];
+        int off[kLoads];
+#pragma unroll
+        for (int j = 0; j < kLoads; ++j) {
+          const int t = t0 + j * nt;
+          const int yu = y0 - 2 + pr;
+          off[j] = (((yu + NSLOT) & (NSLOT - 1)) * WP + pc) * PS + 8 * q;
+          if (t < NP16) {
+            int gy = yu, gx = pc - 2;
+            gy += gy < 0 ? H : (gy >= H ? -H : 0);
+            gx += gx < 0 ? W : (gx >= W ? -W : 0);
+            if constexpr (XF32) {
+              static_assert(CI == 8, "one FP32 input channel pads to 8");
+              const float v = reinterpret_cast<const float*>(x)[((size_t)b * H + gy) * W + gx];
+              vp[j] = u32x4{to_bf16(v), 0u, 0u, 0u};
+            } else {
+              vp[j] = *reinterpret_cast<const u32x4*>(
+                  &x[(((size_t)b * H + gy) * W + gx) * CI + 8 * q]);
+            }
+          }
+          pc += cstep;
+          while (pc >= WP) {
+            pc -= WP;
+            ++pr;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kLoads; ++j) {
+          const int t = t0 + j * nt;
+          if (t < NP16) *reinterpret_cast<u32x4*>(&sp[off[j]]) = vp[j];
+        }
+      }
+    }
+    __syncthreads();
+    const int ybase = (y0 - 2 + NSLOT) & (NSLOT - 1);  // slot of patch row 0
+    for (int tile = wid; tile < R * nchunk; tile += nw) {
+      const int r = tile / nchunk, n0 = (tile % nchunk) * 32;
+      f32x16 acc = {};
+#pragma unroll 5
+      for (int k0 = 0; k0 + 16 <= K; k0 += 16) {
+        const int k = k0 + 8 * g;
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+        const int tap = k / CI, ci = k % CI;
+        const int dy = tap / 5, dx = tap % 5;
+        const int slot = (ybase + r + dy) & (NSLOT - 1);
+        const bf16x8 bb =
+            *reinterpret_cast<const bf16x8*>(&sp[(slot * WP + n0 + m + dx) * PS + ci]);
+        acc = mfma16_bf16(a, bb, acc);
+      }
+      if constexpr (K % 16 != 0) {
+        const int k = K - 8;
+        bf16x8 a = {}, bb = {};
+        if (g == 0) {
+          const int tap = k / CI, ci = k % CI;
+          const int slot = (ybase + r + tap / 5) & (NSLOT - 1);
+          a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+          bb = *reinterpret_cast<const bf16x8*>(&sp[(slot * WP + n0 + m + tap % 5) * PS + ci]);
+        }
+        acc = mfma16_bf16(a, bb, acc);
+      }
+      const size_t pix = ((size_t)b * H + y0 + r) * W + n0 + m;
+      if (FUSE != 1) {
+        float* yo = y + pix * co_out;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int co = 8 * (i >> 2) + 4 * g + (i & 3);
+          if (co < co_out) yo[co] = acc[i] + bias[co];
+        }
+      }
+      if (FUSE != 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int co = 8 * j + 4 * g;
+          uint16_t h[4];
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq) h[qq] = to_bf16(gelu(acc[4 * j + qq] + bias[co + qq]));
+          uint2 v;
+          v.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+          v.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+          *reinterpret_cast<uint2*>(&a_out[pix * 32 + co]) = v;
+        }
+      }
+    }
+  }
+}
+
 // wt[c][24 - t][o] = w[o][t][c] for o < co, c < ci; wt has 32 rows and
 // CO_PAD input channels (zero-padded).
 static __global__ void k_conv_flip(const uint16_t* w, int ci, int co, int co_pad, uint16_t* wt) {

@@ -66,7 +66,10 @@ def conv_gpu(x, w, b, co_out):
 
 
 @pytest.mark.parametrize("ci,shape", [(8, (2, 8, 32)), (32, (2, 8, 32)), (32, (1, 16, 128)),
-                                      (8, (3, 12, 64))])
+                                      (8, (3, 12, 64)),
+                                      # more 4-row blocks than two per CU: the
+                                      # persistent form with its ring of patch rows
+                                      (32, (20, 128, 32)), (8, (40, 64, 32))])
 def test_conv_forward(ci, shape):
     rng = np.random.default_rng(1)
     x = rounded(rng.standard_normal(shape + (ci,)))
@@ -194,12 +197,13 @@ def test_training_step_gradients_and_descent():
 
 
 @pytest.mark.parametrize("ci", [8, 32])
-def test_fused_activation_epilogue_is_bit_exact(ci):
+@pytest.mark.parametrize("shape", [(2, 8, 64), (20, 128, 32)])
+def test_fused_activation_epilogue_is_bit_exact(ci, shape):
     """fsx_cnn_conv_fwd_act (GELU in the convolution epilogue) writes the
     same pre-activations and BF16 activations as fsx_cnn_conv_fwd followed
-    by fsx_cnn_act."""
+    by fsx_cnn_act (also in the persistent form of large batches)."""
     rng = np.random.default_rng(8)
-    B_, H, W = 2, 8, 64
+    B_, H, W = shape
     x = dev(bf16_bits(rng.standard_normal((B_, H, W, ci))))
     w = dev(bf16_bits(rng.standard_normal((32, 25, ci)) * 0.2))
     b = dev(rng.standard_normal(32).astype(np.float32))
