@@ -112,6 +112,7 @@ SIGNATURES = {
     "fsx_cnn_probe16": [P, P, P, P],
     "fsx_cnn_conv_fwd": [P, c_int, c_int, c_int, c_int, P, P, c_int, P, P],
     "fsx_cnn_conv_fwd_act": [P, c_int, c_int, c_int, c_int, P, P, P, P, P],
+    "fsx_cnn_conv_bwd_act": [P, c_int, c_int, c_int, c_int, P, P, P, P, P],
     "fsx_cnn_conv_flip": [P, c_int, c_int, c_int, P, P],
     "fsx_cnn_wgrad_workspace": [c_int, c_int, c_int, ctypes.POINTER(ctypes.c_longlong)],
     "fsx_cnn_conv_wgrad": [

@@ -195,12 +195,11 @@ class CNNClosure:
             wt = torch.empty((32, 25, co_pad), dtype=torch.int16, device="cuda")
             _lib.call("fsx_cnn_conv_flip", _lib.dptr(self.wb[l]), ci, self.cout[l], co_pad,
                       _lib.dptr(wt), s)
-            g = torch.empty((B, H, W, 32), **dev)
-            _lib.call("fsx_cnn_conv_fwd", _lib.dptr(d), B, H, W, co_pad, _lib.dptr(wt),
-                      _lib.dptr(self.zero_bias), 32, _lib.dptr(g), s)
-            d = torch.empty((B, H, W, 32), dtype=torch.int16, device="cuda")
-            _lib.call("fsx_cnn_act_bwd", _lib.dptr(g), 32, _lib.dptr(pres[l - 1]), npix, 32,
-                      _lib.dptr(d), s)
+            # and through the previous layer's GELU, in the convolution epilogue
+            dn = torch.empty((B, H, W, 32), dtype=torch.int16, device="cuda")
+            _lib.call("fsx_cnn_conv_bwd_act", _lib.dptr(d), B, H, W, co_pad, _lib.dptr(wt),
+                      _lib.dptr(self.zero_bias), _lib.dptr(pres[l - 1]), _lib.dptr(dn), s)
+            d = dn
         self.step_count += 1
         lr = self.lr if lr is None else lr
         for i, (p, gr) in enumerate(zip(self.w + self.b, grads_w + grads_b)):

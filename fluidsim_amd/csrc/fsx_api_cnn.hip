@@ -14,7 +14,8 @@ int cnn_grid(long long n) {
 }
 
 // fuse: 0 -> FP32 pre-activations y; 1 -> BF16 GELU activations a (32
-// channels, co_out = 32); 2 -> both
+// channels, co_out = 32); 2 -> both; 3 -> a = BF16((conv + bias) GELU'(y))
+// with y the layer's pre-activations (read; co_out = 32)
 template <int CI, int FUSE, bool XF32>
 void conv_fwd_launch(unsigned nblk, size_t lds, hipStream_t s, const uint16_t* x,
                      const uint16_t* w, const float* bias, float* y, int H, int W, int co_out,
@@ -48,7 +49,7 @@ void conv_fwd_launch(unsigned nblk, size_t lds, hipStream_t s, const uint16_t* x
 int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, const float* bias,
                  int co_out, float* y, hipStream_t s, int fuse = 0, void* a = nullptr,
                  bool x_f32 = false) {
-  if (!x || !w || !bias || B < 1 || fuse < 0 || fuse > 2) return fail(FSX_ERR_ARG, "bad argument");
+  if (!x || !w || !bias || B < 1 || fuse < 0 || fuse > 3) return fail(FSX_ERR_ARG, "bad argument");
   if ((fuse != 1 && !y) || (fuse != 0 && !a)) return fail(FSX_ERR_ARG, "bad argument");
   if (!cnn_channels_ok(ci)) return fail(FSX_ERR_ARG, "input channels must be 8 or 32");
   if (co_out < 1 || co_out > 32) return fail(FSX_ERR_ARG, "output channels must be 1..32");
@@ -84,6 +85,8 @@ int cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w, cons
   FSX_CONV_FWD(32, 0, false)
   FSX_CONV_FWD(32, 1, false)
   FSX_CONV_FWD(32, 2, false)
+  FSX_CONV_FWD(8, 3, false)
+  FSX_CONV_FWD(32, 3, false)
 #undef FSX_CONV_FWD
   FSX_HIP(hipGetLastError());
   return tm_post(s);
@@ -242,6 +245,13 @@ int fsx_cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w,
 int fsx_cnn_conv_fwd_act(const void* x, int B, int H, int W, int ci, const void* w,
                          const float* bias, float* y, void* a, void* stream) {
   return cnn_conv_fwd(x, B, H, W, ci, w, bias, 32, y, (hipStream_t)stream, y ? 2 : 1, a);
+}
+
+int fsx_cnn_conv_bwd_act(const void* x, int B, int H, int W, int ci, const void* w,
+                         const float* bias, const float* pre, void* d, void* stream) {
+  if (!pre || !d) return fail(FSX_ERR_ARG, "bad argument");
+  return cnn_conv_fwd(x, B, H, W, ci, w, bias, 32, const_cast<float*>(pre),
+                      (hipStream_t)stream, 3, d);
 }
 
 int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void* stream) {

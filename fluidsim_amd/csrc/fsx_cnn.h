@@ -111,11 +111,36 @@ __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
   return (size_t)2 * (32 * cnn_kp(ci) + (size_t)(R + 4) * (W + 4) * cnn_ps(ci));
 }
 
+// Epilogue of FUSE = 3 (input gradient of a hidden layer through the
+// activation): a lane's 16 outputs (channels 8 (i / 4) + 4 g + i % 4 of
+// pixel pix) become d = BF16((acc + bias) GELU'(pre)) with the layer's
+// pre-activations pre[pix][32], the value k_cnn_act_bwd computes from the
+// FP32 gradient this replaces.
+__device__ __forceinline__ void conv_bwd_act_store(const f32x16& acc, const float* bias,
+                                                   const float* pre, uint16_t* d, size_t pix,
+                                                   int g) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int co = 8 * j + 4 * g;
+    const float4 p = *reinterpret_cast<const float4*>(&pre[pix * 32 + co]);
+    const float pv[4] = {p.x, p.y, p.z, p.w};
+    uint16_t h[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) h[q] = to_bf16((acc[4 * j + q] + bias[co + q]) * gelu_grad(pv[q]));
+    uint2 v;
+    v.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    v.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+    *reinterpret_cast<uint2*>(&d[pix * 32 + co]) = v;
+  }
+}
+
 // One workgroup (4 waves) per (image, R output rows); each wave computes
 // 32 output channels x 32 pixels tiles.
 // Epilogue: FUSE = 0 writes the FP32 pre-activations y (co_out channels);
 // FUSE = 1 writes the BF16 GELU activations a[p][32] of a hidden layer
-// instead (co_out = 32); FUSE = 2 writes both.
+// instead (co_out = 32); FUSE = 2 writes both; FUSE = 3 (input gradient)
+// reads the pre-activations from y and writes BF16 (acc + bias) GELU'(y)
+// to a (co_out = 32).
 // XF32 (CI = 8 only): x is one FP32 channel x[B][H][W], rounded to BF16 and
 // zero-padded to 8 channels while it is staged.
 // NT threads: 256 (4 waves) for one-row blocks, 512 (8 waves, two tiles
@@ -218,6 +243,10 @@ static __global__ __launch_bounds__(NT) void k_conv_fwd(const uint16_t* __restri
       acc = mfma16_bf16(a, bb, acc);
     }
     const size_t pix = ((size_t)b * H + y0 + r) * W + n0 + m;
+    if constexpr (FUSE == 3) {
+      conv_bwd_act_store(acc, bias, y, a_out, pix, g);
+      continue;
+    }
     if (FUSE != 1) {
       float* yo = y + pix * co_out;
 #pragma unroll
@@ -381,6 +410,10 @@ static __global__ __launch_bounds__(512) void k_conv_fwd_run(
         acc = mfma16_bf16(a, bb, acc);
       }
       const size_t pix = ((size_t)b * H + y0 + r) * W + n0 + m;
+      if constexpr (FUSE == 3) {
+        conv_bwd_act_store(acc, bias, y, a_out, pix, g);
+        continue;
+      }
       if (FUSE != 1) {
         float* yo = y + pix * co_out;
 #pragma unroll

@@ -401,6 +401,10 @@ int fsx_graph_destroy(fsx_graph g);
  * fsx_cnn_conv_fwd_act: a hidden layer (32 outputs) with the activation in
  *   the epilogue: a = BF16(GELU(y)) (32 channels) and, when y is not NULL,
  *   the FP32 pre-activations y the backward pass needs.
+ * fsx_cnn_conv_bwd_act: the input gradient of a hidden layer through its
+ *   activation in one pass: d = BF16((conv(x, w) + bias) * GELU'(pre))
+ *   (32 channels) with pre the layer's FP32 pre-activations, equal to
+ *   fsx_cnn_conv_fwd into an FP32 buffer followed by fsx_cnn_act_bwd.
  * fsx_cnn_conv_flip: wt[c][24 - t][o] = w[o][t][c] (co_pad = 8 or 32
  *   input channels of wt), so fsx_cnn_conv_fwd(dy, wt) is the input gradient.
  * fsx_cnn_conv_wgrad: dw[o][t][c] = sum_p dy[p][o] x[p + d(t)][c] and
@@ -428,6 +432,8 @@ int fsx_cnn_conv_fwd(const void* x, int B, int H, int W, int ci, const void* w,
                      const float* bias, int co_out, float* y, void* stream);
 int fsx_cnn_conv_fwd_act(const void* x, int B, int H, int W, int ci, const void* w,
                          const float* bias, float* y, void* a, void* stream);
+int fsx_cnn_conv_bwd_act(const void* x, int B, int H, int W, int ci, const void* w,
+                         const float* bias, const float* pre, void* d, void* stream);
 int fsx_cnn_conv_flip(const void* w, int ci, int co, int co_pad, void* wt, void* stream);
 int fsx_cnn_wgrad_workspace(int B, int H, int ci, long long* bytes);
 int fsx_cnn_conv_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, int W,

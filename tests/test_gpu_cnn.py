@@ -224,6 +224,35 @@ def test_fused_activation_epilogue_is_bit_exact(ci, shape):
     assert torch.equal(a0, a1) and torch.equal(a0, a2)
 
 
+@pytest.mark.parametrize("ci", [8, 32])
+@pytest.mark.parametrize("shape", [(2, 8, 64), (20, 128, 32)])
+def test_fused_input_gradient_epilogue_is_bit_exact(ci, shape):
+    """fsx_cnn_conv_bwd_act (the input gradient taken through the previous
+    layer's GELU in the convolution epilogue) writes the same BF16 gradient
+    as fsx_cnn_conv_fwd into FP32 followed by fsx_cnn_act_bwd (also in the
+    persistent form of large batches), with a zero and a nonzero bias."""
+    rng = np.random.default_rng(11)
+    B_, H, W = shape
+    x = dev(bf16_bits(rng.standard_normal((B_, H, W, ci))))
+    w = dev(bf16_bits(rng.standard_normal((32, 25, ci)) * 0.2))
+    pre = dev(rng.standard_normal((B_, H, W, 32)).astype(np.float32) * 2)
+    s = _lib.stream_ptr()
+    for b in (torch.zeros(32, device="cuda"), dev(rng.standard_normal(32).astype(np.float32))):
+        g = torch.empty((B_, H, W, 32), dtype=torch.float32, device="cuda")
+        d0 = torch.empty((B_, H, W, 32), dtype=torch.int16, device="cuda")
+        _lib.call("fsx_cnn_conv_fwd", _lib.dptr(x), B_, H, W, ci, _lib.dptr(w), _lib.dptr(b),
+                  32, _lib.dptr(g), s)
+        _lib.call("fsx_cnn_act_bwd", _lib.dptr(g), 32, _lib.dptr(pre), B_ * H * W, 32,
+                  _lib.dptr(d0), s)
+        d1 = torch.zeros_like(d0)
+        _lib.call("fsx_cnn_conv_bwd_act", _lib.dptr(x), B_, H, W, ci, _lib.dptr(w), _lib.dptr(b),
+                  _lib.dptr(pre), _lib.dptr(d1), s)
+        assert torch.equal(d0, d1)
+    with pytest.raises(ValueError):
+        _lib.call("fsx_cnn_conv_bwd_act", _lib.dptr(x), B_, H, W, ci, _lib.dptr(w),
+                  _lib.dptr(b), None, _lib.dptr(d1), s)
+
+
 def test_network_forward_matches_layer_by_layer():
     """fsx_cnn_forward (fused hidden layers) equals the training path's
     layer-by-layer forward bit for bit."""
