@@ -900,34 +900,50 @@ void k_xpass(Op op) {
     // c2r pre-processing, one thread per pair (k, M - k):
     // Z[k] = (X[k] + conj X[M-k]) + i conj(W^k) (X[k] - conj X[M-k]),
     // W = exp(-2 pi i / nx); only the real parts of X[0] and X[M] are used;
-    // the k = 0 thread also does the self-paired k = M/2.
+    // the k = 0 thread also does the self-paired k = M/2.  The loads of
+    // kU consecutive pairs of a thread are issued before any is used, so
+    // their latencies overlap.
+    constexpr int kU = 4;
     const int n_in = (d.nf_in << d.logRB) << logH;
-    for (int idx = tid; idx < n_in; idx += kT) {
-      const int k = idx & (H - 1);
-      const int lr = idx >> logH;
-      const int f = lr >> d.logRB;
-      const int row = row0 + (lr & (d.RB - 1));
-      cpx<T>* Z = buf + lr * ldr;
-      cpx<T> Xk = zero<T>(), Xm = zero<T>(), Xh = zero<T>();
-      if (row < d.nrows) {
-        const cpx<T>* src = op.in + f * d.ifs + (long long)row * d.ld_in;
-        if (k < d.mx_in) Xk = src[k];
-        if (M - k < d.mx_in) Xm = src[M - k];
-        if (k == 0 && H < d.mx_in) Xh = src[H];
+    for (int base = tid; base < n_in; base += kU * kT) {
+      cpx<T> Xk[kU], Xm[kU], Xh[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int idx = base + u * kT;
+        Xk[u] = zero<T>();
+        Xm[u] = zero<T>();
+        Xh[u] = zero<T>();
+        const int k = idx & (H - 1);
+        const int lr = idx >> logH;
+        const int row = row0 + (lr & (d.RB - 1));
+        if (idx < n_in && row < d.nrows) {
+          const cpx<T>* src = op.in + (lr >> d.logRB) * d.ifs + (long long)row * d.ld_in;
+          if (k < d.mx_in) Xk[u] = src[k];
+          if (M - k < d.mx_in) Xm[u] = src[M - k];
+          if (k == 0 && H < d.mx_in) Xh[u] = src[H];
+        }
       }
-      if (k == 0) {
-        Xk.y = T(0);
-        Xm.y = T(0);
-      }
-      const cpx<T> w = s_tw[k];
-      const cpx<T> dk = Xk - conj(Xm), dm = Xm - conj(Xk);
-      Z[lpad(k)] = (Xk + conj(Xm)) + mul_i(conj(w) * dk);
-      if (k == 0) {
-        const cpx<T> dh = Xh - conj(Xh);
-        Z[lpad(H)] = (Xh + conj(Xh)) + mul_i(conj(s_tw[H]) * dh);
-      } else {
-        // conj(W^(M-k)) = -W^k
-        Z[lpad(M - k)] = (Xm + conj(Xk)) - mul_i(w * dm);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int idx = base + u * kT;
+        if (idx >= n_in) break;
+        const int k = idx & (H - 1);
+        const int lr = idx >> logH;
+        cpx<T>* Z = buf + lr * ldr;
+        if (k == 0) {
+          Xk[u].y = T(0);
+          Xm[u].y = T(0);
+        }
+        const cpx<T> w = s_tw[k];
+        const cpx<T> dk = Xk[u] - conj(Xm[u]), dm = Xm[u] - conj(Xk[u]);
+        Z[lpad(k)] = (Xk[u] + conj(Xm[u])) + mul_i(conj(w) * dk);
+        if (k == 0) {
+          const cpx<T> dh = Xh[u] - conj(Xh[u]);
+          Z[lpad(H)] = (Xh[u] + conj(Xh[u])) + mul_i(conj(s_tw[H]) * dh);
+        } else {
+          // conj(W^(M-k)) = -W^k
+          Z[lpad(M - k)] = (Xm[u] + conj(Xk[u])) - mul_i(w * dm);
+        }
       }
     }
     __syncthreads();
