@@ -125,6 +125,10 @@ SIGNATURES = {
     "fsx_cnn_adam": [
         P, P, P, P, c_int, c_double, c_double, c_double, c_double, c_int, P, P,
     ],
+    "fsx_cnn_adam_multi": [
+        c_int, P, P, P, P, ctypes.POINTER(c_int), P, c_double, c_double, c_double, c_double,
+        c_int, P,
+    ],
     "fsx_cnn_to_bf16": [P, c_int, P, P],
     "fsx_cnn_forward": [
         c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), P, P, P, c_int, c_int, c_int,

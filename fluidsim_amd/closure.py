@@ -202,11 +202,21 @@ class CNNClosure:
             d = dn
         self.step_count += 1
         lr = self.lr if lr is None else lr
-        for i, (p, gr) in enumerate(zip(self.w + self.b, grads_w + grads_b)):
-            wb = self.wb[i] if i < self.nlayers else None
-            _lib.call("fsx_cnn_adam", _lib.dptr(p), _lib.dptr(gr), _lib.dptr(self.m1[i]),
-                      _lib.dptr(self.m2[i]), p.numel(), float(lr), 0.9, 0.999, 1e-8,
-                      self.step_count, _lib.dptr(wb) if wb is not None else None, s)
+        # the weight and bias tensors, up to 16 per launch
+        params = self.w + self.b
+        grads = grads_w + grads_b
+        wbs = self.wb + [None] * self.nlayers
+        for c0 in range(0, len(params), 16):
+            sl = slice(c0, c0 + 16)
+            nt = len(params[sl])
+
+            def ptrs(ts):
+                return (ctypes.c_void_p * nt)(
+                    *[t.data_ptr() if t is not None else None for t in ts[sl]])
+
+            _lib.call("fsx_cnn_adam_multi", nt, ptrs(params), ptrs(grads), ptrs(self.m1),
+                      ptrs(self.m2), (ctypes.c_int * nt)(*[p.numel() for p in params[sl]]),
+                      ptrs(wbs), float(lr), 0.9, 0.999, 1e-8, self.step_count, s)
         self.last_grads = (grads_w, grads_b)
         return loss.value if want_loss else None
 

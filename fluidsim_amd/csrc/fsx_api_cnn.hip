@@ -171,10 +171,9 @@ int cnn_wgrad(const void* x, int ci, const void* dy, int co_in, int B, int H, in
 #undef FSX_WG
   FSX_HIP(hipGetLastError());
   const int n = 32 * kCnnTaps * ci;
-  hipLaunchKernelGGL(k_cnn_reduce, dim3((n + 63) / 64), dim3(256), 0, s, part, groups, n, dw);
-  FSX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cnn_reduce, dim3(1), dim3(256), 0, s, part + (size_t)groups * n, groups,
-                     32, dbias);
+  const int nb1 = (n + 63) / 64;  // then one workgroup for the 32 bias sums
+  hipLaunchKernelGGL(k_cnn_reduce, dim3(nb1 + 1), dim3(256), 0, s, part, groups, n, dw, nb1,
+                     part + (size_t)groups * n, 32, dbias);
   FSX_HIP(hipGetLastError());
   return 0;
 }
@@ -343,6 +342,31 @@ int fsx_cnn_adam(float* w, const float* g, float* m1, float* m2, int n, double l
   hipLaunchKernelGGL(k_cnn_adam, dim3(cnn_grid(n)), dim3(256), 0, (hipStream_t)stream, w, g,
                      m1, m2, n, (float)lr, (float)b1, (float)b2, (float)eps, c1, c2,
                      reinterpret_cast<uint16_t*>(wb));
+  FSX_HIP(hipGetLastError());
+  return 0;
+}
+
+int fsx_cnn_adam_multi(int count, float* const* w, const float* const* g, float* const* m1,
+                       float* const* m2, const int* n, void* const* wb, double lr, double b1,
+                       double b2, double eps, int step, void* stream) {
+  if (count < 1 || count > kAdamTensors || !w || !g || !m1 || !m2 || !n || step < 1)
+    return fail(FSX_ERR_ARG, "bad argument (1 to 16 tensors)");
+  AdamTable tb{};
+  tb.count = count;
+  tb.blk[0] = 0;
+  for (int t = 0; t < count; ++t) {
+    if (!w[t] || !g[t] || !m1[t] || !m2[t] || n[t] < 1) return fail(FSX_ERR_ARG, "bad argument");
+    tb.w[t] = w[t];
+    tb.g[t] = g[t];
+    tb.m1[t] = m1[t];
+    tb.m2[t] = m2[t];
+    tb.wb[t] = wb ? reinterpret_cast<uint16_t*>(wb[t]) : nullptr;
+    tb.n[t] = n[t];
+    tb.blk[t + 1] = tb.blk[t] + cnn_grid(n[t]);  // k_cnn_adam's grid
+  }
+  const float c1 = (float)(1.0 - std::pow(b1, step)), c2 = (float)(1.0 - std::pow(b2, step));
+  hipLaunchKernelGGL(k_cnn_adam_multi, dim3(tb.blk[count]), dim3(256), 0, (hipStream_t)stream,
+                     tb, (float)lr, (float)b1, (float)b2, (float)eps, c1, c2);
   FSX_HIP(hipGetLastError());
   return 0;
 }

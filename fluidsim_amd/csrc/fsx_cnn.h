@@ -727,12 +727,22 @@ void k_conv_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ d
 // consecutive outputs; wave w sums the groups w, w + 4, ... into 8
 // interleaved partial sums (8 loads in flight per lane), which are combined
 // in a fixed order, then the 4 waves in a fixed order: deterministic, the
-// same result for every launch. Launch with (n + 63) / 64 workgroups of 256.
+// same result for every launch. Two such sums run in one launch: workgroups
+// below nb1 = (n + 63) / 64 sum part / n into out, the (n2 + 63) / 64
+// after them part2 / n2 into out2 (the weight and the bias partials).
 static __global__ __launch_bounds__(256) void k_cnn_reduce(const float* part, int ngroups, int n,
-                                                    float* out) {
+                                                    float* out, int nb1, const float* part2,
+                                                    int n2, float* out2) {
   __shared__ float red[4][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + lane;
+  int blk = blockIdx.x;
+  if (blk >= nb1) {
+    blk -= nb1;
+    part = part2;
+    n = n2;
+    out = out2;
+  }
+  const int i = blk * 64 + lane;
   float acc[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
@@ -810,6 +820,45 @@ static __global__ __launch_bounds__(256) void k_cnn_mse(const float* out, const 
 static __global__ void k_cnn_adam(float* w, const float* gr, float* m1, float* m2, int n, float lr,
                            float b1, float b2, float eps, float c1, float c2, uint16_t* wb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gi = gr[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    const float wi = w[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = to_bf16(wi);
+  }
+}
+
+// Adam over up to kAdamTensors tensors in one launch, the tensors' pointer
+// table passed by value: tensor t owns the workgroups [blk[t], blk[t + 1])
+// and runs the per-element update of k_cnn_adam over its n[t] elements.
+constexpr int kAdamTensors = 16;
+struct AdamTable {
+  float* w[kAdamTensors];
+  const float* g[kAdamTensors];
+  float* m1[kAdamTensors];
+  float* m2[kAdamTensors];
+  uint16_t* wb[kAdamTensors];
+  int n[kAdamTensors];
+  int blk[kAdamTensors + 1];
+  int count;
+};
+
+static __global__ __launch_bounds__(256) void k_cnn_adam_multi(AdamTable tb, float lr, float b1,
+                                                               float b2, float eps, float c1,
+                                                               float c2) {
+  int t = 0;
+  while (t + 1 < tb.count && (int)blockIdx.x >= tb.blk[t + 1]) ++t;
+  const int nb = tb.blk[t + 1] - tb.blk[t];
+  float* w = tb.w[t];
+  const float* gr = tb.g[t];
+  float* m1 = tb.m1[t];
+  float* m2 = tb.m2[t];
+  uint16_t* wb = tb.wb[t];
+  const int n = tb.n[t];
+  for (int i = (blockIdx.x - tb.blk[t]) * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) {
     const float gi = gr[i];
     const float a = b1 * m1[i] + (1.0f - b1) * gi;
     const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;

@@ -416,6 +416,9 @@ int fsx_graph_destroy(fsx_graph g);
  *   NULL: none) and d = BF16(2 (out - target) / n) in channel 0 of cpad.
  * fsx_cnn_adam: Adam step of n FP32 values (bias-corrected, step >= 1),
  *   optional BF16 copy into wb.
+ * fsx_cnn_adam_multi: fsx_cnn_adam on count (1 to 16) tensors in one
+ *   launch; the arrays (host memory) hold each tensor's device pointers
+ *   and size (wb NULL, or an array whose NULL entries skip the copy).
  * fsx_cnn_to_bf16: BF16 copy of n FP32 values.
  * fsx_cnn_forward: nl layers (GELU between them) in one call, inference;
  *   hidden layers write their BF16 activations from the convolution
@@ -459,6 +462,9 @@ int fsx_cnn_mse(const float* out, const float* target, long long npix, int cpad,
                 double* work, double* host_loss, void* stream);
 int fsx_cnn_adam(float* w, const float* g, float* m1, float* m2, int n, double lr, double b1,
                  double b2, double eps, int step, void* wb, void* stream);
+int fsx_cnn_adam_multi(int count, float* const* w, const float* const* g, float* const* m1,
+                       float* const* m2, const int* n, void* const* wb, double lr, double b1,
+                       double b2, double eps, int step, void* stream);
 int fsx_cnn_to_bf16(const float* in, int n, void* out, void* stream);
 int fsx_cnn_forward(int nl, const int* cin, const int* cout, const void* const* w,
                     const float* const* b, const void* x, int B, int H, int W, float* y,

@@ -56,6 +56,10 @@ CASES = [
     ("fsx_cnn_act_bwd", (P, 8, P, 16, 32, P, None), "bad argument"),
     ("fsx_cnn_mse", (P, P, 0, 8, P, P, None, None), "bad argument"),
     ("fsx_cnn_adam", (P, P, P, P, 10, 1e-3, 0.9, 0.999, 1e-8, 0, None, None), "bad argument"),
+    ("fsx_cnn_adam_multi", (17, ptrs(17), ptrs(17), ptrs(17), ptrs(17), ints(*[8] * 17), None,
+                            1e-3, 0.9, 0.999, 1e-8, 1, None), "1 to 16 tensors"),
+    ("fsx_cnn_adam_multi", (2, ptrs(2), ptrs(2), ptrs(2), ptrs(2), ints(8, 0), None, 1e-3, 0.9,
+                            0.999, 1e-8, 1, None), "bad argument"),
     ("fsx_cnn_to_bf16", (P, 0, P, None), "bad argument"),
     # network in one call: sizes, hidden widths, workspace
     ("fsx_cnn_forward", (0, ints(8), ints(1), ptrs(1), ptrs(1), P, 1, 8, 32, P, P, 1 << 20,

@@ -253,6 +253,49 @@ def test_fused_input_gradient_epilogue_is_bit_exact(ci, shape):
                   _lib.dptr(b), None, _lib.dptr(d1), s)
 
 
+def test_adam_multi_tensor_is_bit_exact():
+    """fsx_cnn_adam_multi (all tensors in one launch) updates weights, both
+    moments and the BF16 copies exactly as one fsx_cnn_adam call per tensor,
+    over two steps, with tensors with and without a BF16 copy and sizes
+    above one grid's worth of workgroups."""
+    import ctypes
+
+    rng = np.random.default_rng(12)
+    sizes = [32 * 25 * 8, 32, 32 * 25 * 32, 1 << 21, 7]
+    s = _lib.stream_ptr()
+
+    def state():
+        return [[dev(rng.standard_normal(n).astype(np.float32)) for n in sizes] for _ in range(3)]
+
+    w0, m10, m20 = state()
+    m10 = [t.abs() * 0.1 for t in m10]
+    m20 = [t.abs() * 0.01 for t in m20]
+    one = [[t.clone() for t in ts] for ts in (w0, m10, m20)]
+    many = [[t.clone() for t in ts] for ts in (w0, m10, m20)]
+    wb1 = [torch.zeros(n, dtype=torch.int16, device="cuda") if i % 2 == 0 else None
+           for i, n in enumerate(sizes)]
+    wb2 = [t.clone() if t is not None else None for t in wb1]
+    nt = len(sizes)
+
+    def ptrs(ts):
+        return (ctypes.c_void_p * nt)(*[t.data_ptr() if t is not None else None for t in ts])
+
+    for step in (1, 2):
+        g = [dev(rng.standard_normal(n).astype(np.float32)) for n in sizes]
+        for i in range(nt):
+            _lib.call("fsx_cnn_adam", _lib.dptr(one[0][i]), _lib.dptr(g[i]), _lib.dptr(one[1][i]),
+                      _lib.dptr(one[2][i]), sizes[i], 1e-3, 0.9, 0.999, 1e-8, step,
+                      _lib.dptr(wb1[i]) if wb1[i] is not None else None, s)
+        _lib.call("fsx_cnn_adam_multi", nt, ptrs(many[0]), ptrs(g), ptrs(many[1]),
+                  ptrs(many[2]), (ctypes.c_int * nt)(*sizes), ptrs(wb2), 1e-3, 0.9, 0.999, 1e-8,
+                  step, s)
+    for a, b in zip(sum(one, []), sum(many, [])):
+        assert torch.equal(a, b)
+    for a, b in zip(wb1, wb2):
+        assert a is None or torch.equal(a, b)
+    assert not torch.equal(one[0][0], w0[0])
+
+
 def test_network_forward_matches_layer_by_layer():
     """fsx_cnn_forward (fused hidden layers) equals the training path's
     layer-by-layer forward bit for bit."""
