@@ -116,6 +116,34 @@ __host__ __device__ inline size_t conv_fwd_lds(int ci, int W, int R) {
 // pixel pix) become d = BF16((acc + bias) GELU'(pre)) with the layer's
 // pre-activations pre[pix][32], the value k_cnn_act_bwd computes from the
 // FP32 gradient this replaces.
+// FP32 outputs acc + bias of one lane's pixel: channels 8 j + 4 g .. + 3
+// (j < 4) are contiguous, one 16-byte store each where co_out is a multiple
+// of 4 and y 16-byte aligned, else one store per channel.
+__device__ __forceinline__ void conv_store_pre(const f32x16& acc, const float* bias, float* y,
+                                               int co_out, size_t pix, int g) {
+  float* yo = y + pix * co_out;
+  if ((co_out & 3) == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int co = 8 * j + 4 * g;
+      if (co < co_out) {
+        float4 v;
+        v.x = acc[4 * j] + bias[co];
+        v.y = acc[4 * j + 1] + bias[co + 1];
+        v.z = acc[4 * j + 2] + bias[co + 2];
+        v.w = acc[4 * j + 3] + bias[co + 3];
+        *reinterpret_cast<float4*>(&yo[co]) = v;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int co = 8 * (i >> 2) + 4 * g + (i & 3);
+      if (co < co_out) yo[co] = acc[i] + bias[co];
+    }
+  }
+}
+
 __device__ __forceinline__ void conv_bwd_act_store(const f32x16& acc, const float* bias,
                                                    const float* pre, uint16_t* d, size_t pix,
                                                    int g) {
@@ -248,12 +276,7 @@ static __global__ __launch_bounds__(NT) void k_conv_fwd(const uint16_t* __restri
       continue;
     }
     if (FUSE != 1) {
-      float* yo = y + pix * co_out;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int co = 8 * (i >> 2) + 4 * g + (i & 3);
-        if (co < co_out) yo[co] = acc[i] + bias[co];
-      }
+      conv_store_pre(acc, bias, y, co_out, pix, g);
     }
     if (FUSE != 0) {
       // channels 8 j + 4 g .. + 3 of this lane are contiguous: one 8-byte store
@@ -384,43 +407,20 @@ static __global__ __launch_bounds__(512) void k_conv_fwd_run(
     }
     __syncthreads();
     const int ybase = (y0 - 2 + NSLOT) & (NSLOT - 1);  // slot of patch row 0
-    for (int tile = wid; tile < R * nchunk; tile += nw) {
-      const int r = tile / nchunk, n0 = (tile % nchunk) * 32;
-      f32x16 acc = {};
-#pragma unroll 5
-      for (int k0 = 0; k0 + 16 <= K; k0 += 16) {
-        const int k = k0 + 8 * g;
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
-        const int tap = k / CI, ci = k % CI;
-        const int dy = tap / 5, dx = tap % 5;
-        const int slot = (ybase + r + dy) & (NSLOT - 1);
-        const bf16x8 bb =
-            *reinterpret_cast<const bf16x8*>(&sp[(slot * WP + n0 + m + dx) * PS + ci]);
-        acc = mfma16_bf16(a, bb, acc);
-      }
-      if constexpr (K % 16 != 0) {
-        const int k = K - 8;
-        bf16x8 a = {}, bb = {};
-        if (g == 0) {
-          const int tap = k / CI, ci = k % CI;
-          const int slot = (ybase + r + tap / 5) & (NSLOT - 1);
-          a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
-          bb = *reinterpret_cast<const bf16x8*>(&sp[(slot * WP + n0 + m + tap % 5) * PS + ci]);
-        }
-        acc = mfma16_bf16(a, bb, acc);
-      }
+    // B operand of tile (r, n0) at k
+    auto bop = [&](int r, int n0, int k) {
+      const int tap = k / CI, ci = k % CI;
+      const int slot = (ybase + r + tap / 5) & (NSLOT - 1);
+      return *reinterpret_cast<const bf16x8*>(&sp[(slot * WP + n0 + m + tap % 5) * PS + ci]);
+    };
+    auto epilogue = [&](const f32x16& acc, int r, int n0) {
       const size_t pix = ((size_t)b * H + y0 + r) * W + n0 + m;
       if constexpr (FUSE == 3) {
         conv_bwd_act_store(acc, bias, y, a_out, pix, g);
-        continue;
+        return;
       }
       if (FUSE != 1) {
-        float* yo = y + pix * co_out;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int co = 8 * (i >> 2) + 4 * g + (i & 3);
-          if (co < co_out) yo[co] = acc[i] + bias[co];
-        }
+        conv_store_pre(acc, bias, y, co_out, pix, g);
       }
       if (FUSE != 0) {
 #pragma unroll
@@ -434,6 +434,57 @@ static __global__ __launch_bounds__(512) void k_conv_fwd_run(
           v.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
           *reinterpret_cast<uint2*>(&a_out[pix * 32 + co]) = v;
         }
+      }
+    };
+    // A wave takes its tiles two at a time (tile and tile + nw), which
+    // share the weight fragment of every k step: 3 LDS reads per 2 MFMAs
+    // instead of 4, and two independent accumulation chains.  Each
+    // accumulator runs the same MFMA sequence as a lone tile.
+    const int ntile = R * nchunk;
+    for (int tile = wid; tile < ntile; tile += 2 * nw) {
+      const int r = tile / nchunk, n0 = (tile % nchunk) * 32;
+      const int tile2 = tile + nw;
+      f32x16 acc = {};
+      if (tile2 < ntile) {
+        const int r2 = tile2 / nchunk, n2 = (tile2 % nchunk) * 32;
+        f32x16 acc2 = {};
+#pragma unroll 5
+        for (int k0 = 0; k0 + 16 <= K; k0 += 16) {
+          const int k = k0 + 8 * g;
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+          acc = mfma16_bf16(a, bop(r, n0, k), acc);
+          acc2 = mfma16_bf16(a, bop(r2, n2, k), acc2);
+        }
+        if constexpr (K % 16 != 0) {
+          const int k = K - 8;
+          bf16x8 a = {}, bb = {}, bb2 = {};
+          if (g == 0) {
+            a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+            bb = bop(r, n0, k);
+            bb2 = bop(r2, n2, k);
+          }
+          acc = mfma16_bf16(a, bb, acc);
+          acc2 = mfma16_bf16(a, bb2, acc2);
+        }
+        epilogue(acc, r, n0);
+        epilogue(acc2, r2, n2);
+      } else {
+#pragma unroll 5
+        for (int k0 = 0; k0 + 16 <= K; k0 += 16) {
+          const int k = k0 + 8 * g;
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+          acc = mfma16_bf16(a, bop(r, n0, k), acc);
+        }
+        if constexpr (K % 16 != 0) {
+          const int k = K - 8;
+          bf16x8 a = {}, bb = {};
+          if (g == 0) {
+            a = *reinterpret_cast<const bf16x8*>(&sw[m * KP + k]);
+            bb = bop(r, n0, k);
+          }
+          acc = mfma16_bf16(a, bb, acc);
+        }
+        epilogue(acc, r, n0);
       }
     }
   }

@@ -75,7 +75,7 @@ def test_conv_forward(ci, shape):
     x = rounded(rng.standard_normal(shape + (ci,)))
     w = rounded(rng.standard_normal((32, 25, ci)) * 0.2)
     b = rng.standard_normal(32)
-    for co in (32, 1):
+    for co in (32, 8, 1):
         got = conv_gpu(x, w, b, co)
         ref = C.conv(x, w, b.astype(np.float32).astype(np.float64), co)
         print(f"conv ci={ci} {shape} co={co}: {relmax(got, ref):.2e}")
