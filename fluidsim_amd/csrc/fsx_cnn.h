@@ -44,13 +44,30 @@ __device__ __forceinline__ float from_bf16(uint16_t h) {
 }
 
 // GELU, exact (erf) form
+// erf x from e = exp(-x^2): Abramowitz & Stegun, Handbook of Mathematical
+// Functions (1964), formula 7.1.26, erf x = 1 - (a1 t + a2 t^2 + a3 t^3 +
+// a4 t^4 + a5 t^5) exp(-x^2) with t = 1 / (1 + p x) for x >= 0 (odd in
+// x), absolute error at most 1.5e-7: one reciprocal and one exponential,
+// which the GELU derivative shares with the Gaussian density.  Far below
+// the BF16 rounding of the activations; the convolution epilogues that
+// evaluate it 16 times per lane per tile were VALU-bound with erff.
+__device__ __forceinline__ float erf_as(float x, float e) {
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, fabsf(x), 1.0f));
+  float p = fmaf(1.061405429f, t, -1.453152027f);
+  p = fmaf(p, t, 1.421413741f);
+  p = fmaf(p, t, -0.284496736f);
+  p = fmaf(p, t, 0.254829592f);
+  return copysignf(fmaf(-p * t, e, 1.0f), x);
+}
 __device__ __forceinline__ float gelu(float v) {
-  return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
+  const float x = v * 0.70710678118654752f;
+  return 0.5f * v * (1.0f + erf_as(x, __expf(-x * x)));
 }
 __device__ __forceinline__ float gelu_grad(float v) {
-  const float cdf = 0.5f * (1.0f + erff(v * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * v * v);
-  return cdf + v * pdf;
+  const float x = v * 0.70710678118654752f;
+  const float e = __expf(-x * x);
+  const float cdf = 0.5f * (1.0f + erf_as(x, e));
+  return cdf + v * (0.39894228040143268f * e);
 }
 
 __device__ __forceinline__ f32x16 mfma_bf16(bf16x4 a, bf16x4 b, f32x16 c) {

@@ -253,6 +253,36 @@ def test_fused_input_gradient_epilogue_is_bit_exact(ci, shape):
                   _lib.dptr(b), None, _lib.dptr(d1), s)
 
 
+def test_activations_match_the_oracle_to_bf16():
+    """fsx_cnn_act (GELU) and fsx_cnn_act_bwd (times GELU') against the FP64
+    oracle: within one BF16 unit in the last place of the rounded oracle
+    value everywhere (the device erf is within 1.5e-7 of erf), and equal to
+    it for nearly all elements at y >= -1; inputs cover the polynomial's whole range,
+    zeros, and large magnitudes where GELU is 0 or the identity."""
+    rng = np.random.default_rng(13)
+    n = 1 << 16
+    y = np.concatenate([rng.standard_normal(n - 8) * 3, [0.0, -0.0, 1e-6, -1e-6, 12, -12, 40, -40]])
+    y = y.astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    s = _lib.stream_ptr()
+    yd, gd = dev(y), dev(g)  # held: the launches only take their addresses
+    a = torch.empty(n, dtype=torch.int16, device="cuda")
+    _lib.call("fsx_cnn_act", _lib.dptr(yd), n, 1, 1, 1, _lib.dptr(a), s)
+    d = torch.empty(n, dtype=torch.int16, device="cuda")
+    _lib.call("fsx_cnn_act_bwd", _lib.dptr(gd), 1, _lib.dptr(yd), n, 1, _lib.dptr(d), s)
+    y64, g64 = y.astype(np.float64), g.astype(np.float64)
+    # 1 + erf(x) cancels in FP32 for large negative x (as with erff): an
+    # absolute allowance of 2^-22 |y| (times |g|) besides the BF16 unit
+    for got_bits, ref, scale in ((a, C.gelu(y64), np.abs(y64)),
+                                 (d, g64 * C.gelu_grad(y64), np.abs(g64 * y64))):
+        got = bf16_value(got_bits.cpu().numpy().view(np.uint16)).astype(np.float64)
+        r = rounded(ref)
+        tol = np.abs(r) * 2.0 ** -7 + scale * 2.0 ** -22
+        assert np.all(np.abs(got - r) <= tol), np.abs(got - r).max()
+        # equal where 1 + erf does not cancel
+        assert np.mean(got[y64 >= -1] == r[y64 >= -1]) > 0.999
+
+
 def test_adam_multi_tensor_is_bit_exact():
     """fsx_cnn_adam_multi (all tensors in one launch) updates weights, both
     moments and the BF16 copies exactly as one fsx_cnn_adam call per tensor,
